@@ -1,0 +1,195 @@
+/*
+ * ispan_hip.h -- C ABI of libispan_hip.so, the MI355X (gfx950) strongly
+ * connected components pipeline of iSpan (SC'18), built for HIP.
+ *
+ * The reference exposes its shared-memory path as C++ functions compiled
+ * into the `ispan` executable (src/scc_common.h:7-33):
+ *     graph_load(fw_beg, fw_csr, bw_beg, bw_csr, avg_time)   src/scc_common.h:7-13
+ *     scc_detection(g, alpha, beta, gamma, theta, T, avg)    src/scc_common.h:15-23
+ *     get_scc_result(scc_id, vert_count)                     src/scc_common.h:25-28
+ *     print_time_result(run_times, avg_time)                 src/scc_common.h:30-33
+ * This header replaces them with plain-pointer entry points.  Inputs are the
+ * reference's int32 CSR pair (src/graph.h:9-31, src/util.h:17-18): fw_beg[V+1],
+ * fw_adj[E] (out-edges) and bw_beg[V+1], bw_adj[E] (in-edges), headerless
+ * little-endian int32 on disk (src/graph.cpp:13-14).
+ *
+ * Output: canonical labels, label[v] = smallest vertex id of v's SCC.  The
+ * reference's raw scc_id encoding (1 giant, -1/-2/-3/-4 trimmed, v >= 2 mice
+ * pivot) is not reproduced; its partition is.
+ *
+ * Errors are negative return codes (the reference calls exit(-1) or asserts,
+ * src/scc_main.cpp:13, src/graph.cpp:21-74).  A context is not thread-safe;
+ * distinct contexts may be used from distinct threads.  Inputs stay owned by
+ * the caller; the library never frees them.
+ */
+#ifndef ISPAN_HIP_H
+#define ISPAN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISPAN_OK 0
+#define ISPAN_ERR_ARG (-1)        /* null pointer, bad size or bad parameter     */
+#define ISPAN_ERR_CSR (-2)        /* malformed CSR (offsets, ids out of range)   */
+#define ISPAN_ERR_NOMEM (-3)      /* device or host allocation failed            */
+#define ISPAN_ERR_HIP (-4)        /* HIP runtime error                           */
+#define ISPAN_ERR_ITER (-5)       /* a fixed-point loop hit its iteration cap    */
+#define ISPAN_ERR_UNLABELLED (-6) /* internal: a vertex left without a label     */
+#define ISPAN_ERR_IO (-7)         /* file missing, short or of the wrong size    */
+#define ISPAN_ERR_NOGRAPH (-8)    /* solve called before a graph was bound       */
+
+/* Tuning knobs.  alpha/beta/gamma/theta keep the meaning of the `ispan`
+ * argv (src/scc_main.cpp:20-25; used at src/fw_bw.h:581,599 and
+ * src/scc_core.cpp:597).  thread_count is accepted for drop-in argv
+ * compatibility and recorded; the GPU path does not use it. */
+typedef struct ispan_params {
+    int32_t alpha;        /* top-down -> bottom-up switch (default 30)              */
+    int32_t beta;         /* bottom-up -> top-down-queue switch (default 200)       */
+    int32_t gamma;        /* level cap for bottom-up (default 10)                   */
+    double theta;         /* trim-1 sweep stop ratio (default 0.1)                  */
+    int32_t trim_times;   /* max trim-1 sweeps after the giant SCC (TRIM_TIMES, 3)  */
+    int32_t thread_count; /* recorded only                                          */
+    int32_t pivot_wrap32; /* 1: reproduce the reference's int32 out*in product      */
+    int32_t timing;       /* 1: per-phase HIP event timing into ispan_stats.ms      */
+    int32_t validate;     /* 1: check CSR structure when a graph is bound           */
+    int32_t wcc_mode;     /* 0: union-find hooking, 1: label propagation + jumping  */
+    int32_t async_batch;  /* top-down levels enqueued per host check (default 8)    */
+    int32_t reserved[5];
+} ispan_params;
+
+/* Per-solve statistics.  ms[] uses the reference's avg_time slot map
+ * (src/load_print.cpp:67-68, src/scc_core.cpp:1038-1052):
+ *   0 trim, 1 elephant SCC (pivot+fw+bw), 2 mice SCC (wcc+mice), 3 total,
+ *   4 trim size_1, 5 trim size_2, 6 pivot, 7 fw bfs, 8 bw bfs, 9 wcc,
+ *   10 mice fw-bw, 13 trim size_3; plus 11 trim-1-first (with queue and
+ *   pivot), 14 trim-1 sweeps.  Filled when params.timing = 1, except ms[3],
+ *   always the host-measured solve time. */
+typedef struct ispan_stats {
+    double ms[16];
+    int64_t fq0;          /* vertices left after trim-1-first                */
+    int64_t pivot;        /* giant-SCC pivot (-1: none)                      */
+    int64_t fw_size;      /* |FW reach set of the pivot|                     */
+    int64_t giant_size;   /* |giant SCC|                                     */
+    int64_t fw_edges;     /* sum of out-degrees over the FW set              */
+    int64_t bw_edges;     /* sum of in-degrees over the giant SCC            */
+    int64_t trim1;        /* vertices labelled by trim-1 (all sweeps)        */
+    int64_t trim2;        /* vertices labelled by the size-2 pattern         */
+    int64_t trim3;        /* vertices labelled by the size-3 patterns        */
+    int64_t n_wcc;        /* WCCs entering the mice phase                    */
+    int64_t mice;         /* vertices entering the mice phase                */
+    int64_t fw_levels;    /* FW traversal levels                             */
+    int64_t bw_levels;    /* BW traversal levels                             */
+    int64_t n_scc;        /* number of SCCs (computed on request)            */
+    int64_t host_syncs;   /* host<->device round trips in this solve         */
+    int64_t reserved[9];
+} ispan_stats;
+
+typedef struct ispan_ctx ispan_ctx;
+
+/* Defaults: alpha 30, beta 200, gamma 10, theta 0.1, trim_times 3
+ * (src/bash_one.sh:3, src/util.h:34), validate 1, everything else 0. */
+void ispan_default_params(ispan_params* p);
+
+/* Human-readable text for an error code. */
+const char* ispan_strerror(int code);
+
+/* Context: device buffers, a HIP stream, pinned readback area.  `device` is
+ * a HIP ordinal.  Replaces the per-call allocation at src/scc_core.cpp:56-92. */
+int ispan_ctx_create(ispan_ctx** out, int device);
+int ispan_ctx_destroy(ispan_ctx* ctx);
+/* Use a caller-owned hipStream_t (passed as void*) for all work; NULL = the
+ * context's own stream. */
+int ispan_ctx_set_stream(ispan_ctx* ctx, void* hip_stream);
+/* Message of the last error seen by this context. */
+const char* ispan_ctx_last_error(const ispan_ctx* ctx);
+
+/* Copy a host CSR pair to the device (the device-side half of graph::graph,
+ * src/graph.cpp:2-91).  Validates when params->validate (or params NULL). */
+int ispan_graph_upload(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj,
+                       const int32_t* bw_beg, const int32_t* bw_adj, const ispan_params* params);
+/* Bind caller-owned device arrays (no copy).  They must stay valid and
+ * unchanged until the next bind/upload or ctx destruction. */
+int ispan_graph_bind_device(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t* d_fw_beg, const int32_t* d_fw_adj,
+                            const int32_t* d_bw_beg, const int32_t* d_bw_adj, const ispan_params* params);
+
+/* One SCC solve of the bound graph (scc_detection, src/scc_core.cpp:27-1094).
+ * Labels stay on the device; d_labels (int32[V], device) receives them when
+ * not NULL.  stats may be NULL. */
+int ispan_solve(ispan_ctx* ctx, const ispan_params* params, int32_t* d_labels, ispan_stats* stats);
+/* Copy the labels of the last solve to host memory (int32[V]). */
+int ispan_labels_download(ispan_ctx* ctx, int32_t* labels);
+/* Device pointer of the labels of the last solve (int32[V], owned by ctx). */
+const int32_t* ispan_labels_device(const ispan_ctx* ctx);
+/* Fill stats->n_scc, fw_edges and bw_edges for the last solve (extra pass). */
+int ispan_solve_stats(ispan_ctx* ctx, ispan_stats* stats);
+
+/* One-shot host-pointer convenience: upload, solve, download (the whole of
+ * scc_detection + the label hook of get_scc_result).  labels: int32[V]. */
+int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
+                        const int32_t* bw_adj, const ispan_params* params, int32_t* labels, ispan_stats* stats);
+
+/* ---- reference file I/O and reporting ---------------------------------- */
+
+/* Host graph as loaded from the four iSpan binaries (graph_load,
+ * src/load_print.cpp:6-22 / graph::graph, src/graph.cpp:2-91):
+ * V = size(fw_begin)/4 - 1, E = size(fw_adjacent)/4. */
+typedef struct ispan_host_graph {
+    int32_t V;
+    int64_t E;
+    int32_t* fw_beg;
+    int32_t* fw_adj;
+    int32_t* bw_beg;
+    int32_t* bw_adj;
+} ispan_host_graph;
+
+int ispan_graph_load(const char* fw_beg_file, const char* fw_adj_file, const char* bw_beg_file,
+                     const char* bw_adj_file, ispan_host_graph* out);
+void ispan_graph_free(ispan_host_graph* g);
+/* Structural check of a host CSR pair (offsets monotone, beg[V] == E, ids in
+ * range, bw the transpose-size of fw).  Returns ISPAN_OK or ISPAN_ERR_CSR. */
+int ispan_csr_check(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
+                    const int32_t* bw_adj);
+
+/* SCC size summary of canonical labels (get_scc_result, src/load_print.cpp:
+ * 24-60, without its phantom entry): largest SCC size, counts of SCCs of
+ * size 1, 2, 3 and larger-than-3-but-not-largest, total SCC count. */
+typedef struct ispan_summary {
+    int64_t largest;
+    int64_t size_1;
+    int64_t size_2;
+    int64_t size_3;
+    int64_t others;
+    int64_t total;
+} ispan_summary;
+int ispan_summarize(int32_t V, const int32_t* labels, ispan_summary* out);
+
+/* ---- synthetic inputs built on the device ------------------------------ */
+
+/* R-MAT (a, b, c, 1-a-b-c) draws m = round(edge_factor * V) at
+ * ceil(log2 V) bit levels from a counter hash of (seed, draw); draws with an
+ * endpoint >= V are dropped, then self-loops and duplicates are removed and
+ * both CSR directions are built on the device.  The caller frees the four
+ * device arrays with ispan_device_free.  (Format producer of
+ * graph_converter/txt_to_bin_{fw,bw}_int.cpp, with dedup.) */
+int ispan_gen_rmat_device(ispan_ctx* ctx, int32_t V, double edge_factor, double a, double b, double c, uint64_t seed,
+                          int32_t** d_fw_beg, int32_t** d_fw_adj, int32_t** d_bw_beg, int32_t** d_bw_adj, int64_t* E);
+/* n x n lattice; each lattice edge bidirectional with probability p_bidir,
+ * else one random direction; plus L/100 random directed edges. */
+int ispan_gen_grid_device(ispan_ctx* ctx, int32_t n, double p_bidir, uint64_t seed, int32_t** d_fw_beg,
+                          int32_t** d_fw_adj, int32_t** d_bw_beg, int32_t** d_bw_adj, int64_t* E);
+/* Edge list (device int32 src[m], dst[m]) -> deduplicated CSR pair. */
+int ispan_build_csr_device(ispan_ctx* ctx, int32_t V, int64_t m, const int32_t* d_src, const int32_t* d_dst,
+                           int32_t** d_fw_beg, int32_t** d_fw_adj, int32_t** d_bw_beg, int32_t** d_bw_adj,
+                           int64_t* E);
+void ispan_device_free(void* d_ptr);
+
+/* Library version string. */
+const char* ispan_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISPAN_HIP_H */

@@ -1,0 +1,356 @@
+"""ispan_amd: MI355X (gfx950) strongly-connected-components pipeline of iSpan.
+
+Python binding over the C ABI of ``ispan_amd/lib/libispan_hip.so``
+(declared in ``include/ispan_hip.h``).  The GPU library is the only compute
+path: importing works without it, but every call that would compute raises
+``NativeLibraryMissing`` when it has not been built.
+
+Mirrors the reference's C++ entry points (src/scc_common.h:7-33):
+
+* ``graph_load(fw_beg, fw_adj, bw_beg, bw_adj)``  -> ``HostGraph``
+* ``scc_detection(g, alpha, beta, gamma, theta, thread_count)`` -> labels
+* ``get_scc_result(labels)`` -> SCC size summary
+plus a ``Solver`` context for device-resident graphs and repeated solves.
+Labels are canonical: ``label[v]`` = smallest vertex id in v's SCC.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libispan_hip.so")
+CLI_PATH = os.path.join(PKG, "bin", "ispan")
+
+ISPAN_OK = 0
+ERRORS = {-1: "ISPAN_ERR_ARG", -2: "ISPAN_ERR_CSR", -3: "ISPAN_ERR_NOMEM", -4: "ISPAN_ERR_HIP",
+          -5: "ISPAN_ERR_ITER", -6: "ISPAN_ERR_UNLABELLED", -7: "ISPAN_ERR_IO", -8: "ISPAN_ERR_NOGRAPH"}
+
+# Every symbol include/ispan_hip.h declares.
+EXPORTS = [
+    "ispan_default_params", "ispan_strerror", "ispan_ctx_create", "ispan_ctx_destroy", "ispan_ctx_set_stream",
+    "ispan_ctx_last_error", "ispan_graph_upload", "ispan_graph_bind_device", "ispan_solve", "ispan_labels_download",
+    "ispan_labels_device", "ispan_solve_stats", "ispan_scc_detection", "ispan_graph_load", "ispan_graph_free",
+    "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
+    "ispan_build_csr_device", "ispan_device_free", "ispan_version",
+]
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class IspanError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+
+
+class Params(C.Structure):
+    """ispan_params (include/ispan_hip.h)."""
+    _fields_ = [("alpha", C.c_int32), ("beta", C.c_int32), ("gamma", C.c_int32), ("theta", C.c_double),
+                ("trim_times", C.c_int32), ("thread_count", C.c_int32), ("pivot_wrap32", C.c_int32),
+                ("timing", C.c_int32), ("validate", C.c_int32), ("wcc_mode", C.c_int32),
+                ("async_batch", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class Stats(C.Structure):
+    """ispan_stats (include/ispan_hip.h)."""
+    _fields_ = [("ms", C.c_double * 16), ("fq0", C.c_int64), ("pivot", C.c_int64), ("fw_size", C.c_int64),
+                ("giant_size", C.c_int64), ("fw_edges", C.c_int64), ("bw_edges", C.c_int64),
+                ("trim1", C.c_int64), ("trim2", C.c_int64), ("trim3", C.c_int64), ("n_wcc", C.c_int64),
+                ("mice", C.c_int64), ("fw_levels", C.c_int64), ("bw_levels", C.c_int64), ("n_scc", C.c_int64),
+                ("host_syncs", C.c_int64), ("reserved", C.c_int64 * 9)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("ms", "reserved")}
+        names = {0: "trim", 1: "elephant", 2: "mice_total", 3: "total", 4: "trim_size_1", 5: "trim_size_2",
+                 6: "pivot", 7: "fw_bfs", 8: "bw_bfs", 9: "wcc", 10: "mice_fw_bw", 11: "trim_1_first",
+                 13: "trim_size_3", 14: "trim_1_sweeps"}
+        d["ms"] = {n: self.ms[i] for i, n in names.items()}
+        return d
+
+
+class HostGraphC(C.Structure):
+    _fields_ = [("V", C.c_int32), ("E", C.c_int64), ("fw_beg", C.POINTER(C.c_int32)),
+                ("fw_adj", C.POINTER(C.c_int32)), ("bw_beg", C.POINTER(C.c_int32)),
+                ("bw_adj", C.POINTER(C.c_int32))]
+
+
+class Summary(C.Structure):
+    _fields_ = [("largest", C.c_int64), ("size_1", C.c_int64), ("size_2", C.c_int64), ("size_3", C.c_int64),
+                ("others", C.c_int64), ("total", C.c_int64)]
+
+
+_lib = None
+_vp = C.c_void_p
+_i32pp = C.POINTER(C.POINTER(C.c_int32))
+
+
+def library_available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    """Load libispan_hip.so (raises NativeLibraryMissing if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryMissing(f"{LIB_PATH} not built; run `python -m ispan_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "ispan_default_params": (None, [C.POINTER(Params)]),
+        "ispan_strerror": (C.c_char_p, [C.c_int]),
+        "ispan_version": (C.c_char_p, []),
+        "ispan_ctx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+        "ispan_ctx_destroy": (C.c_int, [_vp]),
+        "ispan_ctx_set_stream": (C.c_int, [_vp, _vp]),
+        "ispan_ctx_last_error": (C.c_char_p, [_vp]),
+        "ispan_graph_upload": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params)]),
+        "ispan_graph_bind_device": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params)]),
+        "ispan_solve": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(Stats)]),
+        "ispan_labels_download": (C.c_int, [_vp, _vp]),
+        "ispan_labels_device": (_vp, [_vp]),
+        "ispan_solve_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
+        "ispan_scc_detection": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
+                                          C.POINTER(Stats)]),
+        "ispan_graph_load": (C.c_int, [C.c_char_p] * 4 + [C.POINTER(HostGraphC)]),
+        "ispan_graph_free": (None, [C.POINTER(HostGraphC)]),
+        "ispan_csr_check": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]),
+        "ispan_summarize": (C.c_int, [C.c_int32, _vp, C.POINTER(Summary)]),
+        "ispan_gen_rmat_device": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_uint64, _i32pp, _i32pp, _i32pp, _i32pp, C.POINTER(C.c_int64)]),
+        "ispan_gen_grid_device": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_uint64, _i32pp, _i32pp, _i32pp,
+                                            _i32pp, C.POINTER(C.c_int64)]),
+        "ispan_build_csr_device": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _i32pp, _i32pp, _i32pp, _i32pp,
+                                             C.POINTER(C.c_int64)]),
+        "ispan_device_free": (None, [_vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().ispan_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data if a.size else 0
+
+
+def _check(rc: int, ctx=None):
+    if rc != ISPAN_OK:
+        msg = lib().ispan_ctx_last_error(ctx).decode() if ctx else lib().ispan_strerror(rc).decode()
+        raise IspanError(rc, msg)
+
+
+@dataclass
+class HostGraph:
+    """iSpan CSR pair in host memory (src/graph.h:9-31)."""
+    fw_beg: np.ndarray
+    fw_adj: np.ndarray
+    bw_beg: np.ndarray
+    bw_adj: np.ndarray
+
+    @property
+    def V(self) -> int:
+        return int(self.fw_beg.shape[0]) - 1
+
+    @property
+    def E(self) -> int:
+        return int(self.fw_adj.shape[0])
+
+
+def graph_load(fw_beg_file: str, fw_adj_file: str, bw_beg_file: str, bw_adj_file: str) -> HostGraph:
+    """Read the four iSpan binaries (graph_load, src/load_print.cpp:6-22)."""
+    g = HostGraphC()
+    _check(lib().ispan_graph_load(fw_beg_file.encode(), fw_adj_file.encode(), bw_beg_file.encode(),
+                                  bw_adj_file.encode(), C.byref(g)))
+    try:
+        V, E = g.V, g.E
+        out = HostGraph(np.ctypeslib.as_array(g.fw_beg, (V + 1,)).copy(),
+                        np.ctypeslib.as_array(g.fw_adj, (max(E, 1),))[:E].copy(),
+                        np.ctypeslib.as_array(g.bw_beg, (V + 1,)).copy(),
+                        np.ctypeslib.as_array(g.bw_adj, (max(E, 1),))[:E].copy())
+    finally:
+        lib().ispan_graph_free(C.byref(g))
+    return out
+
+
+@dataclass
+class DeviceGraph:
+    """CSR pair produced on the device by the library; free() releases it."""
+    V: int
+    E: int
+    fw_beg: int
+    fw_adj: int
+    bw_beg: int
+    bw_adj: int
+
+    def free(self):
+        for p in (self.fw_beg, self.fw_adj, self.bw_beg, self.bw_adj):
+            if p:
+                lib().ispan_device_free(p)
+        self.fw_beg = self.fw_adj = self.bw_beg = self.bw_adj = 0
+
+    def download(self) -> HostGraph:
+        out = []
+        for p, n in ((self.fw_beg, self.V + 1), (self.fw_adj, self.E), (self.bw_beg, self.V + 1),
+                     (self.bw_adj, self.E)):
+            a = np.empty(n, np.int32)
+            if n:
+                _check(_hiprt().hipMemcpy(a.ctypes.data, p, a.nbytes, 2) and -4)  # hipMemcpyDeviceToHost
+            out.append(a)
+        return HostGraph(*out)
+
+
+_hip = None
+
+
+def _hiprt():
+    """The HIP runtime, for plain device<->host copies of library-owned arrays."""
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise NativeLibraryMissing("libamdhip64.so not found")
+        _hip.hipMemcpy.restype = C.c_int
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _hip
+
+
+class Solver:
+    """One ispan_ctx: a device, a stream, device-resident graph and labels."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = C.c_void_p()
+        _check(lib().ispan_ctx_create(C.byref(self._ctx), device))
+        self.V = -1
+        self.E = 0
+        self._keep = None
+
+    def close(self):
+        if self._ctx:
+            lib().ispan_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def set_stream(self, stream_handle: int | None):
+        _check(lib().ispan_ctx_set_stream(self._ctx, stream_handle or None), self._ctx)
+
+    def upload(self, g: HostGraph, params: Params | None = None):
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)]
+        V, E = int(arrs[0].shape[0]) - 1, int(arrs[1].shape[0])
+        _check(lib().ispan_graph_upload(self._ctx, V, E, *[_ptr(a) for a in arrs],
+                                        C.byref(params) if params else None), self._ctx)
+        self.V, self.E = V, E
+        self._keep = None
+
+    def bind_device(self, V: int, E: int, fw_beg: int, fw_adj: int, bw_beg: int, bw_adj: int,
+                    params: Params | None = None, keep=None):
+        _check(lib().ispan_graph_bind_device(self._ctx, V, E, fw_beg, fw_adj, bw_beg, bw_adj,
+                                             C.byref(params) if params else None), self._ctx)
+        self.V, self.E = V, E
+        self._keep = keep
+
+    def bind(self, dg: DeviceGraph, params: Params | None = None):
+        self.bind_device(dg.V, dg.E, dg.fw_beg, dg.fw_adj, dg.bw_beg, dg.bw_adj, params, keep=dg)
+
+    def solve(self, params: Params | None = None, d_labels: int = 0) -> Stats:
+        st = Stats()
+        _check(lib().ispan_solve(self._ctx, C.byref(params) if params else None, d_labels or None, C.byref(st)),
+               self._ctx)
+        return st
+
+    def solve_stats(self, st: Stats) -> Stats:
+        _check(lib().ispan_solve_stats(self._ctx, C.byref(st)), self._ctx)
+        return st
+
+    def labels(self) -> np.ndarray:
+        out = np.empty(max(self.V, 0), np.int32)
+        _check(lib().ispan_labels_download(self._ctx, _ptr(out) if out.size else None), self._ctx)
+        return out
+
+    def labels_device_ptr(self) -> int:
+        return lib().ispan_labels_device(self._ctx) or 0
+
+    def gen_rmat(self, V: int, edge_factor: float = 16.0, seed: int = 1, a=0.57, b=0.19, c=0.19) -> DeviceGraph:
+        ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]
+        E = C.c_int64()
+        _check(lib().ispan_gen_rmat_device(self._ctx, V, edge_factor, a, b, c, seed, *[C.byref(p) for p in ptrs],
+                                           C.byref(E)), self._ctx)
+        return DeviceGraph(V, E.value, *[C.cast(p, C.c_void_p).value or 0 for p in ptrs])
+
+    def gen_grid(self, n: int, p_bidir: float = 0.6, seed: int = 1) -> DeviceGraph:
+        ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]
+        E = C.c_int64()
+        _check(lib().ispan_gen_grid_device(self._ctx, n, p_bidir, seed, *[C.byref(p) for p in ptrs], C.byref(E)),
+               self._ctx)
+        return DeviceGraph(n * n, E.value, *[C.cast(p, C.c_void_p).value or 0 for p in ptrs])
+
+    def build_csr(self, V: int, m: int, d_src: int, d_dst: int) -> DeviceGraph:
+        ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]
+        E = C.c_int64()
+        _check(lib().ispan_build_csr_device(self._ctx, V, m, d_src, d_dst, *[C.byref(p) for p in ptrs],
+                                            C.byref(E)), self._ctx)
+        return DeviceGraph(V, E.value, *[C.cast(p, C.c_void_p).value or 0 for p in ptrs])
+
+
+def scc_detection(g: HostGraph, alpha: int = 30, beta: int = 200, gamma: int = 10, theta: float = 0.1,
+                  thread_count: int = 1, **kw) -> np.ndarray:
+    """scc_detection (src/scc_core.cpp:27-1094) on the GPU -> canonical labels."""
+    p = default_params(alpha=int(alpha), beta=int(beta), gamma=int(gamma), theta=float(theta),
+                       thread_count=int(thread_count), **kw)
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)]
+    V, E = int(arrs[0].shape[0]) - 1, int(arrs[1].shape[0])
+    lab = np.empty(max(V, 1), np.int32)
+    st = Stats()
+    _check(lib().ispan_scc_detection(V, E, *[_ptr(a) for a in arrs], C.byref(p), _ptr(lab), C.byref(st)))
+    return lab[:V]
+
+
+def get_scc_result(labels: np.ndarray) -> dict:
+    """SCC size summary (get_scc_result, src/load_print.cpp:24-60, without its phantom entry)."""
+    lab = np.ascontiguousarray(labels, np.int32)
+    s = Summary()
+    _check(lib().ispan_summarize(int(lab.shape[0]), _ptr(lab), C.byref(s)))
+    return {k: getattr(s, k) for k, _ in Summary._fields_}
+
+
+def csr_check(g: HostGraph) -> bool:
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)]
+    return lib().ispan_csr_check(g.V, g.E, *[_ptr(a) for a in arrs]) == ISPAN_OK

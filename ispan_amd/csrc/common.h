@@ -1,0 +1,119 @@
+// Internal declarations shared by the kernels and the host-side solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace ispan {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kWave = 64;            // gfx950 wavefront width
+constexpr int kTrimTile = 4096;      // vertices per tile in the first trim sweep
+constexpr int kHubDeg = 4096;        // top-down: degree at which a vertex is split
+constexpr int kHubChunk = 4096;      // edges per hub work item
+constexpr int kBuTileWords = 32;     // bottom-up: bitmap words (x64 vertices) per tile
+constexpr int kWaveBuf = 256;        // per-wave LDS staging for queue pushes
+
+// Device control block: every count a kernel needs to size its work lives
+// here, so kernels launch on fixed grids and the host reads it back only
+// where a decision needs it.
+struct alignas(128) Ctl {
+    int32_t fq_n[2];        // active-vertex queue lengths (ping-pong)
+    int32_t q_n[2];         // traversal frontier lengths (ping-pong by level parity)
+    int32_t hub_n[2];       // hub work items (ping-pong by level parity)
+    int32_t root;           // giant-SCC pivot, -1 if none
+    int32_t giant_min;      // smallest id in the giant SCC
+    int32_t levels;         // traversal levels that had a non-empty frontier
+    int32_t trimmed;        // vertices labelled by the current trim sweep
+    int32_t pattern2;       // vertices labelled by the size-2 pattern
+    int32_t pattern3;       // vertices labelled by the size-3 patterns
+    int32_t n_wcc;          // WCCs in the mice phase
+    int32_t changed;        // fixed-point flag (label propagation)
+    int32_t unlabelled;     // final check
+    uint32_t done_blocks;   // last-block-done counter (pivot reduction)
+    int32_t bad_csr;        // CSR validation failures
+    int32_t pad0;
+    int64_t fw_count;       // |FW set| (popcount of fvis)
+    int64_t fw_edges;       // statistics pass
+    int64_t bw_edges;
+    int64_t n_scc;
+    int32_t pad[2];
+};
+
+struct PivotPart {
+    int64_t prod;
+    int32_t id;
+    int32_t pad;
+};
+
+struct HubItem {
+    int32_t v;
+    int32_t start;  // first edge offset of this chunk
+};
+
+// Grid sizes for grid-stride kernels.
+struct Launch {
+    int grid_v;     // sweeps over V
+    int grid_q;     // sweeps over queues
+    int cu;         // compute units
+};
+
+// ---- kernel launchers (kernels.hip) --------------------------------------
+// All take the stream last and return the hipError_t of the launch.
+hipError_t launch_reset_ctl(Ctl* ctl, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, int grid, hipStream_t s);
+hipError_t launch_trim1_first(const int32_t* fwb, const int32_t* bwb, int32_t V, int32_t* lab, uint64_t* act,
+                              uint64_t* fvis, uint64_t* bvis, int32_t* fq, int32_t* q0, Ctl* ctl, PivotPart* part,
+                              int pivot_wrap32, int grid, hipStream_t s);
+// One top-down level: expand q[par] into q[par^1] (forward, bw == false:
+// out-edges, claim act & ~fvis; backward: in-edges, claim fvis & ~bvis).
+hipError_t launch_td_level(bool bw, const int32_t* beg, const int32_t* adj, const uint64_t* allow, uint64_t* vis,
+                           int32_t* const q[2], HubItem* const hub[2], Ctl* ctl, int par, int grid, hipStream_t s);
+// One bottom-up sweep over the candidates allow & ~vis; pushes every vertex
+// it visits into q[par^1].
+hipError_t launch_bu_level(bool bw, const int32_t* beg, const int32_t* adj, const uint64_t* allow, uint64_t* vis,
+                           int32_t* const q[2], HubItem* const hub[2], Ctl* ctl, int par, int64_t nwords, int grid,
+                           hipStream_t s);
+hipError_t launch_bw_init(uint64_t* bvis, int32_t* q0, Ctl* ctl, hipStream_t s);
+hipError_t launch_popcount(const uint64_t* bm, int64_t nwords, int64_t* out, int grid, hipStream_t s);
+// act &= ~bvis, giant labels, fq[1] rebuilt from act.
+hipError_t launch_post_giant(int32_t V, int32_t* lab, uint64_t* act, const uint64_t* bvis, int32_t* fq, Ctl* ctl,
+                             int grid, hipStream_t s);
+// trim-1 sweep fq[sel] -> fq[sel^1]
+hipError_t launch_trim1_sweep(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa,
+                              int32_t* lab, uint64_t* act, int32_t* const fq[2], Ctl* ctl, int sel, int grid,
+                              hipStream_t s);
+hipError_t launch_pattern_trim(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa,
+                               int32_t* lab, int32_t* stamp, const uint64_t* act, const int32_t* fq, Ctl* ctl,
+                               int sel, int grid, hipStream_t s);
+hipError_t launch_pattern_commit(int32_t* lab, uint64_t* act, const int32_t* stamp, const int32_t* fq, Ctl* ctl,
+                                 int sel, int grid, hipStream_t s);
+hipError_t launch_wcc_uf(const int32_t* fwb, const int32_t* fwa, const uint64_t* act, const int32_t* fq,
+                         int32_t* parent, Ctl* ctl, int sel, int grid, hipStream_t s);
+hipError_t launch_uf_init(const int32_t* fq, int32_t* parent, Ctl* ctl, int sel, int grid, hipStream_t s);
+hipError_t launch_lp_round(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa,
+                           const uint64_t* act, const int32_t* fq, int32_t* color, Ctl* ctl, int sel, int grid,
+                           hipStream_t s);
+hipError_t launch_mice_keys(const int32_t* fq, const int32_t* wcc, int32_t* stamp, uint64_t* keys, Ctl* ctl, int sel,
+                            int grid, hipStream_t s);
+hipError_t launch_mice_starts(const uint64_t* keys, int32_t n, int32_t* starts, Ctl* ctl, int grid, hipStream_t s);
+hipError_t launch_mice(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa, int32_t* lab,
+                       int32_t* stamp, const uint64_t* keys, int32_t n, const int32_t* starts, Ctl* ctl,
+                       int32_t* qbuf, int grid, hipStream_t s);
+hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, Ctl* ctl, int grid, hipStream_t s);
+hipError_t launch_stats(const int32_t* fwb, const int32_t* bwb, const int32_t* lab, const uint64_t* fvis,
+                        const uint64_t* bvis, int32_t V, Ctl* ctl, int grid, hipStream_t s);
+hipError_t launch_check_csr(const int32_t* beg, const int32_t* adj, int32_t V, int64_t E, Ctl* ctl, int grid,
+                            hipStream_t s);
+
+// ---- graph building (gen.hip).  Return 0, -1 bad argument, -2 bad edge,
+// -3 out of memory, -4 HIP error (message in err).
+int gen_rmat_device(hipStream_t s, int32_t V, double ef, double a, double b, double c, uint64_t seed, int32_t** fwb,
+                    int32_t** fwa, int32_t** bwb, int32_t** bwa, int64_t* E, std::string& err);
+int gen_grid_device(hipStream_t s, int32_t n, double p_bidir, uint64_t seed, int32_t** fwb, int32_t** fwa,
+                    int32_t** bwb, int32_t** bwa, int64_t* E, std::string& err);
+int build_csr_device(hipStream_t s, int32_t V, int64_t m, const int32_t* src, const int32_t* dst, int32_t** fwb,
+                     int32_t** fwa, int32_t** bwb, int32_t** bwa, int64_t* E, std::string& err);
+
+}  // namespace ispan

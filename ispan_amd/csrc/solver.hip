@@ -1,0 +1,752 @@
+// Host side of libispan_hip.so: the context (device buffers, stream, pinned
+// readback block), the SCC phase sequence of scc_detection
+// (src/scc_core.cpp:27-1094) driven over the kernels in kernels.hip, and the
+// C ABI declared in include/ispan_hip.h.
+#include <string.h>
+
+#include <cstring>
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/time.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ispan_hip.h"
+#include "common.h"
+
+using namespace ispan;
+
+struct ispan_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int cu = 0;
+    int grid = 0;
+    std::string err;
+
+    // bound graph
+    int32_t V = -1;
+    int64_t E = 0;
+    const int32_t* fwb = nullptr;
+    const int32_t* fwa = nullptr;
+    const int32_t* bwb = nullptr;
+    const int32_t* bwa = nullptr;
+    int32_t* owned[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t nwords = 0;
+
+    // solve state (capacity cap_V / cap_E)
+    int32_t cap_V = -1;
+    int64_t cap_E = -1;
+    int32_t* lab = nullptr;
+    uint64_t* act = nullptr;
+    uint64_t* fvis = nullptr;
+    uint64_t* bvis = nullptr;
+    int32_t* fq[2] = {nullptr, nullptr};
+    int32_t* q[2] = {nullptr, nullptr};
+    HubItem* hub[2] = {nullptr, nullptr};
+    int32_t* parent = nullptr;
+    int32_t* stamp = nullptr;
+    uint64_t* keys = nullptr;
+    uint64_t* keys_alt = nullptr;
+    int32_t* starts = nullptr;
+    int32_t* qbuf = nullptr;
+    PivotPart* part = nullptr;
+    Ctl* ctl = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    Ctl* h_ctl = nullptr;  // pinned
+    hipEvent_t ev[16] = {};
+    bool have_solution = false;
+};
+
+namespace {
+
+const char* const kVersion = "ispan_hip 0.1 (gfx950)";
+
+double wall_ms() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec * 1e3 + tv.tv_usec * 1e-3;
+}
+
+#define CK(call)                                                                                  \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                         \
+            return e_ == hipErrorOutOfMemory ? ISPAN_ERR_NOMEM : ISPAN_ERR_HIP;                   \
+        }                                                                                         \
+    } while (0)
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_solve_buffers(ispan_ctx* ctx) {
+    dfree(ctx->lab);
+    dfree(ctx->act);
+    dfree(ctx->fvis);
+    dfree(ctx->bvis);
+    for (int i = 0; i < 2; ++i) {
+        dfree(ctx->fq[i]);
+        dfree(ctx->q[i]);
+        dfree(ctx->hub[i]);
+    }
+    dfree(ctx->parent);
+    dfree(ctx->stamp);
+    dfree(ctx->keys);
+    dfree(ctx->keys_alt);
+    dfree(ctx->starts);
+    dfree(ctx->qbuf);
+    dfree(ctx->part);
+    if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
+    ctx->sort_tmp = nullptr;
+    ctx->sort_tmp_bytes = 0;
+    ctx->cap_V = -1;
+    ctx->cap_E = -1;
+    ctx->have_solution = false;
+}
+
+void free_graph(ispan_ctx* ctx) {
+    for (auto& p : ctx->owned) dfree(p);
+    ctx->fwb = ctx->fwa = ctx->bwb = ctx->bwa = nullptr;
+    ctx->V = -1;
+    ctx->E = 0;
+    ctx->have_solution = false;
+}
+
+int64_t hub_capacity(int64_t E) { return 2 * (E / kHubChunk) + 64; }
+
+int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
+    if (V <= ctx->cap_V && E <= ctx->cap_E) return ISPAN_OK;
+    free_solve_buffers(ctx);
+    const size_t nv = (size_t)std::max(V, 1) + 1;
+    const size_t nw = (size_t)((V + 63) / 64) + 1;
+    CK(hipMalloc(&ctx->lab, nv * 4));
+    CK(hipMalloc(&ctx->act, nw * 8));
+    CK(hipMalloc(&ctx->fvis, nw * 8));
+    CK(hipMalloc(&ctx->bvis, nw * 8));
+    for (int i = 0; i < 2; ++i) {
+        CK(hipMalloc(&ctx->fq[i], nv * 4));
+        CK(hipMalloc(&ctx->q[i], nv * 4));
+        CK(hipMalloc(&ctx->hub[i], (size_t)hub_capacity(E) * sizeof(HubItem)));
+    }
+    CK(hipMalloc(&ctx->parent, nv * 4));
+    CK(hipMalloc(&ctx->stamp, nv * 4));
+    CK(hipMalloc(&ctx->keys, nv * 8));
+    CK(hipMalloc(&ctx->keys_alt, nv * 8));
+    CK(hipMalloc(&ctx->starts, nv * 4));
+    CK(hipMalloc(&ctx->qbuf, nv * 4));
+    CK(hipMalloc(&ctx->part, (size_t)ctx->grid * sizeof(PivotPart)));
+    size_t bytes = 0;
+    CK(rocprim::radix_sort_keys(nullptr, bytes, ctx->keys, ctx->keys_alt, (size_t)nv, 0, 64, ctx->stream));
+    CK(hipMalloc(&ctx->sort_tmp, bytes + 256));
+    ctx->sort_tmp_bytes = bytes + 256;
+    ctx->cap_V = V;
+    ctx->cap_E = E;
+    return ISPAN_OK;
+}
+
+int readback(ispan_ctx* ctx, int64_t* syncs) {
+    CK(hipMemcpyAsync(ctx->h_ctl, ctx->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (syncs) ++*syncs;
+    return ISPAN_OK;
+}
+
+// One direction-optimising traversal (fw_bfs_fq_queue, src/fw_bw.h:396-698;
+// bw_bfs_fq_queue, src/fw_bw.h:944-1248) over the kernels: sync top-down
+// (mode 0), Rsync bottom-up (mode 1), then top-down to exhaustion (mode 2),
+// with the reference's switch rules on alpha, beta and gamma.  set_size is
+// the reference's fq_size for this traversal (the active queue for FW, the
+// FW set for BW).  Top-down levels of mode 2 are enqueued async_batch at a
+// time between host checks (a level with an empty input does nothing).
+int traverse(ispan_ctx* ctx, bool bw, int64_t set_size, const ispan_params* P, double avg_degree, int64_t* syncs) {
+    hipStream_t s = ctx->stream;
+    const int32_t* td_beg = bw ? ctx->bwb : ctx->fwb;
+    const int32_t* td_adj = bw ? ctx->bwa : ctx->fwa;
+    const int32_t* bu_beg = bw ? ctx->fwb : ctx->bwb;
+    const int32_t* bu_adj = bw ? ctx->fwa : ctx->bwa;
+    const uint64_t* allow = bw ? ctx->fvis : ctx->act;
+    uint64_t* vis = bw ? ctx->bvis : ctx->fvis;
+    int par = 0, mode = 0;
+    int64_t visited = 0;
+    int32_t level = 0;
+    const int batch = P->async_batch > 0 ? P->async_batch : 8;
+    const int64_t level_cap = (int64_t)ctx->V + 2;
+    while (true) {
+        if (mode == 2) {
+            for (int b = 0; b < batch; ++b) {
+                CK(launch_td_level(bw, td_beg, td_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->grid, s));
+                par ^= 1;
+            }
+            level += batch;
+            int rc = readback(ctx, syncs);
+            if (rc) return rc;
+            if (ctx->h_ctl->q_n[par] == 0) break;
+            if (level > level_cap) {
+                ctx->err = "traversal exceeded V levels";
+                return ISPAN_ERR_ITER;
+            }
+            continue;
+        }
+        if (mode == 0)
+            CK(launch_td_level(bw, td_beg, td_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->grid, s));
+        else
+            CK(launch_bu_level(bw, bu_beg, bu_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->nwords,
+                               ctx->grid, s));
+        int rc = readback(ctx, syncs);
+        if (rc) return rc;
+        const int64_t frontier = ctx->h_ctl->q_n[par ^ 1];
+        visited += frontier;
+        if (frontier == 0) break;
+        if (mode == 0) {
+            const double ef = (double)frontier * avg_degree;
+            const double er = (double)(set_size - visited) * avg_degree;
+            if (er / P->alpha < ef) mode = 1;
+        } else if ((set_size * 1.0 / P->beta) > frontier || level > P->gamma) {
+            mode = 2;  // BU pushed its finds into q[par^1]: the next input
+        }
+        par ^= 1;
+        ++level;
+        if (level > level_cap) {
+            ctx->err = "traversal exceeded V levels";
+            return ISPAN_ERR_ITER;
+        }
+    }
+    return ISPAN_OK;
+}
+
+void rec(ispan_ctx* ctx, const ispan_params* P, int i) {
+    if (P->timing) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
+float ev_ms(ispan_ctx* ctx, int a, int b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.f;
+    return ms;
+}
+
+int check_bound(ispan_ctx* ctx, const ispan_params* P) {
+    if (!P || !P->validate) return ISPAN_OK;
+    CK(launch_reset_ctl(ctx->ctl, ctx->stream));
+    if (ctx->V > 0) {
+        CK(launch_check_csr(ctx->fwb, ctx->fwa, ctx->V, ctx->E, ctx->ctl, ctx->grid, ctx->stream));
+        CK(launch_check_csr(ctx->bwb, ctx->bwa, ctx->V, ctx->E, ctx->ctl, ctx->grid, ctx->stream));
+    }
+    int rc = readback(ctx, nullptr);
+    if (rc) return rc;
+    if (ctx->h_ctl->bad_csr) {
+        ctx->err = "CSR check failed (" + std::to_string(ctx->h_ctl->bad_csr) + " violations)";
+        return ISPAN_ERR_CSR;
+    }
+    return ISPAN_OK;
+}
+
+int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_stats* st) {
+    const double t0 = wall_ms();
+    hipStream_t s = ctx->stream;
+    const int32_t V = ctx->V;
+    const int64_t E = ctx->E;
+    ispan_stats local{};
+    int64_t syncs = 0;
+    int rc = ensure_buffers(ctx, V, E);
+    if (rc) return rc;
+    ctx->have_solution = false;
+    const double avg_degree = V > 0 ? (double)E / V : 0.0;
+    const int G = ctx->grid;
+
+    CK(launch_reset_ctl(ctx->ctl, s));
+    rec(ctx, P, 0);
+    if (V == 0) {
+        CK(hipStreamSynchronize(s));
+        ctx->have_solution = true;
+        if (st) *st = local;
+        return ISPAN_OK;
+    }
+    // P1-P3: trim-1-first, active queue, pivot
+    CK(launch_trim1_first(ctx->fwb, ctx->bwb, V, ctx->lab, ctx->act, ctx->fvis, ctx->bvis, ctx->fq[0], ctx->q[0],
+                          ctx->ctl, ctx->part, P->pivot_wrap32, G, s));
+    rec(ctx, P, 1);
+    if ((rc = readback(ctx, &syncs))) return rc;
+    const int64_t fq0 = ctx->h_ctl->fq_n[0];
+    const int32_t root = ctx->h_ctl->root;
+    local.fq0 = fq0;
+    local.pivot = root;
+    rec(ctx, P, 2);
+    int64_t fw_count = 0;
+    if (root >= 0) {
+        // P4 forward traversal
+        if ((rc = traverse(ctx, false, fq0, P, avg_degree, &syncs))) return rc;
+        local.fw_levels = ctx->h_ctl->levels;
+        CK(launch_popcount(ctx->fvis, ctx->nwords, &ctx->ctl->fw_count, G, s));
+        rec(ctx, P, 3);
+        if ((rc = readback(ctx, &syncs))) return rc;
+        fw_count = ctx->h_ctl->fw_count;
+        // P5-P6 backward traversal inside the FW set
+        CK(launch_bw_init(ctx->bvis, ctx->q[0], ctx->ctl, s));
+        if ((rc = traverse(ctx, true, fw_count, P, avg_degree, &syncs))) return rc;
+        local.bw_levels = ctx->h_ctl->levels;
+    } else {
+        rec(ctx, P, 3);
+    }
+    local.fw_size = fw_count;
+    CK(launch_post_giant(V, ctx->lab, ctx->act, ctx->bvis, ctx->fq[1], ctx->ctl, G, s));
+    rec(ctx, P, 4);
+    if ((rc = readback(ctx, &syncs))) return rc;
+    int sel = 1;
+    int64_t n = ctx->h_ctl->fq_n[1];
+    local.giant_size = root >= 0 ? fq0 - n : 0;  // giant = removed by post_giant
+    int64_t trim1 = 0;
+
+    // P7 trim-1 sweeps with the theta stop rule (src/scc_core.cpp:562-602):
+    // the first comparison is against the pre-giant queue, as there.
+    {
+        int64_t prev = fq0;
+        int it = 0;
+        while (prev > 0 && it < P->trim_times) {
+            if (n == 0) break;
+            CK(launch_trim1_sweep(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->act, ctx->fq, ctx->ctl, sel,
+                                  G, s));
+            sel ^= 1;
+            if ((rc = readback(ctx, &syncs))) return rc;
+            const int64_t nn = ctx->h_ctl->fq_n[sel];
+            trim1 += ctx->h_ctl->trimmed;
+            n = nn;
+            if (prev - nn < prev * P->theta) break;
+            prev = nn;
+            ++it;
+        }
+    }
+    rec(ctx, P, 5);
+    int64_t pat2 = 0, pat3 = 0;
+    if (n > 0) {
+        // P8-P10 size-2/3 pattern trims
+        CK(launch_pattern_trim(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->act, ctx->fq[sel],
+                               ctx->ctl, sel, G, s));
+        CK(launch_pattern_commit(ctx->lab, ctx->act, ctx->stamp, ctx->fq[sel], ctx->ctl, sel, G, s));
+        rec(ctx, P, 6);
+        // P11 one trim-1 sweep
+        CK(launch_trim1_sweep(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->act, ctx->fq, ctx->ctl, sel, G,
+                              s));
+        sel ^= 1;
+        rec(ctx, P, 7);
+        if ((rc = readback(ctx, &syncs))) return rc;
+        n = ctx->h_ctl->fq_n[sel];
+        trim1 += ctx->h_ctl->trimmed;
+        pat2 = ctx->h_ctl->pattern2;
+        pat3 = ctx->h_ctl->pattern3;
+        if (n > 0) {
+            // P12 WCC of the remaining subgraph
+            if (P->wcc_mode == 1) {
+                CK(launch_uf_init(ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
+                int it = 0;
+                for (;; ++it) {
+                    if (it > 4 * 1024 * 1024) {
+                        ctx->err = "label propagation did not converge";
+                        return ISPAN_ERR_ITER;
+                    }
+                    CK(launch_lp_round(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->act, ctx->fq[sel], ctx->parent,
+                                       ctx->ctl, sel, G, s));
+                    if ((rc = readback(ctx, &syncs))) return rc;
+                    if (!ctx->h_ctl->changed) break;
+                }
+            } else {
+                CK(launch_wcc_uf(ctx->fwb, ctx->fwa, ctx->act, ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
+            }
+            rec(ctx, P, 8);
+            // P13-P14 mice: group by WCC, FW-BW per WCC
+            CK(launch_mice_keys(ctx->fq[sel], ctx->parent, ctx->stamp, ctx->keys, ctx->ctl, sel, G, s));
+            int end_bit = 32;
+            while (end_bit < 64 && ((int64_t)1 << (end_bit - 32)) < (int64_t)V) ++end_bit;
+            size_t bytes = ctx->sort_tmp_bytes;
+            CK(rocprim::radix_sort_keys(ctx->sort_tmp, bytes, ctx->keys, ctx->keys_alt, (size_t)n, 0,
+                                        (unsigned)end_bit, s));
+            CK(launch_mice_starts(ctx->keys_alt, (int32_t)n, ctx->starts, ctx->ctl, G, s));
+            CK(launch_mice(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->keys_alt, (int32_t)n,
+                           ctx->starts, ctx->ctl, ctx->qbuf, G, s));
+            local.mice = n;
+        } else {
+            rec(ctx, P, 8);
+        }
+    } else {
+        rec(ctx, P, 6);
+        rec(ctx, P, 7);
+        rec(ctx, P, 8);
+    }
+    rec(ctx, P, 9);
+    CK(launch_count_unlabelled(ctx->lab, V, ctx->ctl, G, s));
+    if ((rc = readback(ctx, &syncs))) return rc;
+    if (ctx->h_ctl->unlabelled) {
+        ctx->err = std::to_string(ctx->h_ctl->unlabelled) + " vertices left unlabelled";
+        return ISPAN_ERR_UNLABELLED;
+    }
+    if (d_labels && d_labels != ctx->lab) {
+        CK(hipMemcpyAsync(d_labels, ctx->lab, (size_t)V * 4, hipMemcpyDeviceToDevice, s));
+        CK(hipStreamSynchronize(s));
+    }
+    local.n_wcc = ctx->h_ctl->n_wcc;
+    local.trim1 = trim1 + (V - fq0);
+    local.trim2 = pat2;
+    local.trim3 = pat3;
+    local.host_syncs = syncs;
+    local.ms[3] = wall_ms() - t0;
+    if (P->timing) {
+        // event map: 0 start, 1 trim-1-first+queue+pivot, 2 pivot readback,
+        // 3 FW done, 4 BW + giant relabel done, 5 trim-1 sweeps, 6 size-2/3,
+        // 7 last trim-1 sweep, 8 WCC, 9 mice
+        const double first = ev_ms(ctx, 0, 1), fw = ev_ms(ctx, 2, 3), bw = ev_ms(ctx, 3, 4),
+                     t1 = ev_ms(ctx, 4, 5), t23 = ev_ms(ctx, 5, 6), t1b = ev_ms(ctx, 6, 7), wcc = ev_ms(ctx, 7, 8),
+                     mice = ev_ms(ctx, 8, 9);
+        local.ms[11] = first;
+        local.ms[14] = t1 + t1b;
+        local.ms[4] = first + t1 + t1b;
+        local.ms[5] = t23;
+        local.ms[13] = 0.0;  // size-3 patterns run fused with size-2 (ms[5])
+        local.ms[6] = 0.0;   // pivot reduction is fused into trim-1-first (ms[11])
+        local.ms[7] = fw;
+        local.ms[8] = bw;
+        local.ms[9] = wcc;
+        local.ms[10] = mice;
+        local.ms[0] = local.ms[4] + local.ms[5] + local.ms[13];
+        local.ms[1] = fw + bw;
+        local.ms[2] = wcc + mice;
+    }
+    ctx->have_solution = true;
+    if (st) *st = local;
+    return ISPAN_OK;
+}
+
+int bind_common(ispan_ctx* ctx, int32_t V, int64_t E) {
+    ctx->V = V;
+    ctx->E = E;
+    ctx->nwords = ((int64_t)V + 63) / 64;
+    return ISPAN_OK;
+}
+
+int64_t file_size(const char* path) {
+    struct stat sb;
+    if (stat(path, &sb) != 0) return -1;
+    return (int64_t)sb.st_size;
+}
+
+int read_i32(const char* path, int64_t count, int32_t* out) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return ISPAN_ERR_IO;
+    size_t got = fread(out, 4, (size_t)count, f);
+    fclose(f);
+    return got == (size_t)count ? ISPAN_OK : ISPAN_ERR_IO;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+
+extern "C" {
+
+void ispan_default_params(ispan_params* p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->alpha = 30;
+    p->beta = 200;
+    p->gamma = 10;
+    p->theta = 0.1;
+    p->trim_times = 3;
+    p->thread_count = 1;
+    p->validate = 1;
+    p->async_batch = 8;
+}
+
+const char* ispan_strerror(int code) {
+    switch (code) {
+        case ISPAN_OK: return "ok";
+        case ISPAN_ERR_ARG: return "invalid argument";
+        case ISPAN_ERR_CSR: return "malformed CSR";
+        case ISPAN_ERR_NOMEM: return "out of memory";
+        case ISPAN_ERR_HIP: return "HIP runtime error";
+        case ISPAN_ERR_ITER: return "iteration cap reached";
+        case ISPAN_ERR_UNLABELLED: return "vertices left unlabelled";
+        case ISPAN_ERR_IO: return "file I/O error";
+        case ISPAN_ERR_NOGRAPH: return "no graph bound";
+        default: return "unknown error";
+    }
+}
+
+const char* ispan_version(void) { return kVersion; }
+
+int ispan_ctx_create(ispan_ctx** out, int device) {
+    if (!out) return ISPAN_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return ISPAN_ERR_ARG;
+    ispan_ctx* ctx = new ispan_ctx();
+    ctx->device = device;
+    auto fail = [&](int code) {
+        ispan_ctx_destroy(ctx);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(ISPAN_ERR_HIP);
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) return fail(ISPAN_ERR_HIP);
+    ctx->stream = ctx->own_stream;
+    if (hipDeviceGetAttribute(&ctx->cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return fail(ISPAN_ERR_HIP);
+    ctx->grid = std::max(ctx->cu, 1) * 8;
+    if (hipMalloc(&ctx->ctl, sizeof(Ctl)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
+    if (hipHostMalloc(&ctx->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
+    for (auto& e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(ISPAN_ERR_HIP);
+    *out = ctx;
+    return ISPAN_OK;
+}
+
+int ispan_ctx_destroy(ispan_ctx* ctx) {
+    if (!ctx) return ISPAN_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_solve_buffers(ctx);
+    free_graph(ctx);
+    dfree(ctx->ctl);
+    if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return ISPAN_OK;
+}
+
+int ispan_ctx_set_stream(ispan_ctx* ctx, void* hip_stream) {
+    if (!ctx) return ISPAN_ERR_ARG;
+    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    return ISPAN_OK;
+}
+
+const char* ispan_ctx_last_error(const ispan_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ispan_graph_upload(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj,
+                       const int32_t* bw_beg, const int32_t* bw_adj, const ispan_params* params) {
+    if (!ctx || V < 0 || E < 0 || E > INT32_MAX || !fw_beg || !bw_beg || (E > 0 && (!fw_adj || !bw_adj)))
+        return ISPAN_ERR_ARG;
+    if (fw_beg[0] != 0 || bw_beg[0] != 0 || (int64_t)fw_beg[V] != E || (int64_t)bw_beg[V] != E) {
+        ctx->err = "beg[0] must be 0 and beg[V] must equal E";
+        return ISPAN_ERR_CSR;
+    }
+    CK(hipSetDevice(ctx->device));
+    free_graph(ctx);
+    const size_t nb = ((size_t)V + 1) * 4, na = (size_t)std::max<int64_t>(E, 1) * 4;
+    CK(hipMalloc(&ctx->owned[0], nb));
+    CK(hipMalloc(&ctx->owned[1], na));
+    CK(hipMalloc(&ctx->owned[2], nb));
+    CK(hipMalloc(&ctx->owned[3], na));
+    CK(hipMemcpyAsync(ctx->owned[0], fw_beg, nb, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->owned[2], bw_beg, nb, hipMemcpyHostToDevice, ctx->stream));
+    if (E > 0) {
+        CK(hipMemcpyAsync(ctx->owned[1], fw_adj, (size_t)E * 4, hipMemcpyHostToDevice, ctx->stream));
+        CK(hipMemcpyAsync(ctx->owned[3], bw_adj, (size_t)E * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->fwb = ctx->owned[0];
+    ctx->fwa = ctx->owned[1];
+    ctx->bwb = ctx->owned[2];
+    ctx->bwa = ctx->owned[3];
+    bind_common(ctx, V, E);
+    ispan_params d;
+    ispan_default_params(&d);
+    int rc = check_bound(ctx, params ? params : &d);
+    if (rc) free_graph(ctx);
+    return rc;
+}
+
+int ispan_graph_bind_device(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t* d_fw_beg, const int32_t* d_fw_adj,
+                            const int32_t* d_bw_beg, const int32_t* d_bw_adj, const ispan_params* params) {
+    if (!ctx || V < 0 || E < 0 || E > INT32_MAX || !d_fw_beg || !d_bw_beg || (E > 0 && (!d_fw_adj || !d_bw_adj)))
+        return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    free_graph(ctx);
+    ctx->fwb = d_fw_beg;
+    ctx->fwa = d_fw_adj;
+    ctx->bwb = d_bw_beg;
+    ctx->bwa = d_bw_adj;
+    bind_common(ctx, V, E);
+    ispan_params d;
+    ispan_default_params(&d);
+    int rc = check_bound(ctx, params ? params : &d);
+    if (rc) free_graph(ctx);
+    return rc;
+}
+
+int ispan_solve(ispan_ctx* ctx, const ispan_params* params, int32_t* d_labels, ispan_stats* stats) {
+    if (!ctx) return ISPAN_ERR_ARG;
+    if (ctx->V < 0) return ISPAN_ERR_NOGRAPH;
+    ispan_params d;
+    ispan_default_params(&d);
+    const ispan_params* P = params ? params : &d;
+    if (P->alpha <= 0 || P->beta <= 0 || P->trim_times < 0) return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    return solve_impl(ctx, P, d_labels, stats);
+}
+
+int ispan_labels_download(ispan_ctx* ctx, int32_t* labels) {
+    if (!ctx || (!labels && ctx->V > 0)) return ISPAN_ERR_ARG;
+    if (!ctx->have_solution) return ISPAN_ERR_NOGRAPH;
+    if (ctx->V == 0) return ISPAN_OK;
+    CK(hipMemcpyAsync(labels, ctx->lab, (size_t)ctx->V * 4, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ISPAN_OK;
+}
+
+const int32_t* ispan_labels_device(const ispan_ctx* ctx) {
+    return (ctx && ctx->have_solution) ? ctx->lab : nullptr;
+}
+
+int ispan_solve_stats(ispan_ctx* ctx, ispan_stats* st) {
+    if (!ctx || !st) return ISPAN_ERR_ARG;
+    if (!ctx->have_solution) return ISPAN_ERR_NOGRAPH;
+    if (ctx->V == 0) return ISPAN_OK;
+    CK(launch_reset_ctl(ctx->ctl, ctx->stream));
+    CK(launch_stats(ctx->fwb, ctx->bwb, ctx->lab, ctx->fvis, ctx->bvis, ctx->V, ctx->ctl, ctx->grid, ctx->stream));
+    int rc = readback(ctx, nullptr);
+    if (rc) return rc;
+    st->fw_edges = ctx->h_ctl->fw_edges;
+    st->bw_edges = ctx->h_ctl->bw_edges;
+    st->n_scc = ctx->h_ctl->n_scc;
+    return ISPAN_OK;
+}
+
+int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
+                        const int32_t* bw_adj, const ispan_params* params, int32_t* labels, ispan_stats* stats) {
+    if (!labels && V > 0) return ISPAN_ERR_ARG;
+    ispan_ctx* ctx = nullptr;
+    int rc = ispan_ctx_create(&ctx, 0);
+    if (rc) return rc;
+    rc = ispan_graph_upload(ctx, V, E, fw_beg, fw_adj, bw_beg, bw_adj, params);
+    if (!rc) rc = ispan_solve(ctx, params, nullptr, stats);
+    if (!rc) rc = ispan_labels_download(ctx, labels);
+    ispan_ctx_destroy(ctx);
+    return rc;
+}
+
+int ispan_graph_load(const char* fw_beg_file, const char* fw_adj_file, const char* bw_beg_file,
+                     const char* bw_adj_file, ispan_host_graph* out) {
+    if (!fw_beg_file || !fw_adj_file || !bw_beg_file || !bw_adj_file || !out) return ISPAN_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    const int64_t sb = file_size(fw_beg_file), sa = file_size(fw_adj_file);
+    const int64_t sb2 = file_size(bw_beg_file), sa2 = file_size(bw_adj_file);
+    if (sb < 4 || sa < 0 || sb2 != sb || sa2 != sa || sb % 4 || sa % 4) return ISPAN_ERR_IO;
+    const int64_t V = sb / 4 - 1, E = sa / 4;
+    if (V > INT32_MAX - 1 || E > INT32_MAX) return ISPAN_ERR_IO;
+    out->V = (int32_t)V;
+    out->E = E;
+    out->fw_beg = (int32_t*)malloc((size_t)(V + 1) * 4);
+    out->bw_beg = (int32_t*)malloc((size_t)(V + 1) * 4);
+    out->fw_adj = (int32_t*)malloc((size_t)std::max<int64_t>(E, 1) * 4);
+    out->bw_adj = (int32_t*)malloc((size_t)std::max<int64_t>(E, 1) * 4);
+    if (!out->fw_beg || !out->bw_beg || !out->fw_adj || !out->bw_adj) {
+        ispan_graph_free(out);
+        return ISPAN_ERR_NOMEM;
+    }
+    int rc = read_i32(fw_beg_file, V + 1, out->fw_beg);
+    if (!rc) rc = read_i32(fw_adj_file, E, out->fw_adj);
+    if (!rc) rc = read_i32(bw_beg_file, V + 1, out->bw_beg);
+    if (!rc) rc = read_i32(bw_adj_file, E, out->bw_adj);
+    if (rc) ispan_graph_free(out);
+    return rc;
+}
+
+void ispan_graph_free(ispan_host_graph* g) {
+    if (!g) return;
+    free(g->fw_beg);
+    free(g->fw_adj);
+    free(g->bw_beg);
+    free(g->bw_adj);
+    memset(g, 0, sizeof(*g));
+}
+
+int ispan_csr_check(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
+                    const int32_t* bw_adj) {
+    if (V < 0 || E < 0 || !fw_beg || !bw_beg || (E > 0 && (!fw_adj || !bw_adj))) return ISPAN_ERR_ARG;
+    for (int d = 0; d < 2; ++d) {
+        const int32_t* b = d ? bw_beg : fw_beg;
+        const int32_t* a = d ? bw_adj : fw_adj;
+        if (b[0] != 0 || (int64_t)b[V] != E) return ISPAN_ERR_CSR;
+        for (int32_t v = 0; v < V; ++v)
+            if (b[v] > b[v + 1]) return ISPAN_ERR_CSR;
+        for (int64_t k = 0; k < E; ++k)
+            if (a[k] < 0 || a[k] >= V) return ISPAN_ERR_CSR;
+    }
+    return ISPAN_OK;
+}
+
+int ispan_summarize(int32_t V, const int32_t* labels, ispan_summary* out) {
+    if (V < 0 || !out || (V > 0 && !labels)) return ISPAN_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    std::vector<int32_t> size((size_t)V, 0);
+    for (int32_t v = 0; v < V; ++v) {
+        const int32_t l = labels[v];
+        if (l < 0 || l > v) return ISPAN_ERR_ARG;  // canonical: label <= own id
+        ++size[l];
+    }
+    int64_t largest = 0;
+    for (int32_t v = 0; v < V; ++v) largest = std::max<int64_t>(largest, size[v]);
+    bool largest_taken = false;
+    for (int32_t v = 0; v < V; ++v) {
+        const int32_t c = size[v];
+        if (!c) continue;
+        ++out->total;
+        if (c == largest && !largest_taken && largest > 3) {
+            largest_taken = true;
+            continue;
+        }
+        if (c == 1) ++out->size_1;
+        else if (c == 2) ++out->size_2;
+        else if (c == 3) ++out->size_3;
+        else ++out->others;
+    }
+    out->largest = largest;
+    return ISPAN_OK;
+}
+
+static int gen_rc(int rc) {
+    return rc == 0 ? ISPAN_OK : rc == -1 ? ISPAN_ERR_ARG : rc == -2 ? ISPAN_ERR_CSR : rc == -3 ? ISPAN_ERR_NOMEM
+                                                                                               : ISPAN_ERR_HIP;
+}
+
+int ispan_gen_rmat_device(ispan_ctx* ctx, int32_t V, double edge_factor, double a, double b, double c, uint64_t seed,
+                          int32_t** d_fw_beg, int32_t** d_fw_adj, int32_t** d_bw_beg, int32_t** d_bw_adj, int64_t* E) {
+    if (!ctx || !d_fw_beg || !d_fw_adj || !d_bw_beg || !d_bw_adj || !E) return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    return gen_rc(gen_rmat_device(ctx->stream, V, edge_factor, a, b, c, seed, d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj,
+                                  E, ctx->err));
+}
+
+int ispan_gen_grid_device(ispan_ctx* ctx, int32_t n, double p_bidir, uint64_t seed, int32_t** d_fw_beg,
+                          int32_t** d_fw_adj, int32_t** d_bw_beg, int32_t** d_bw_adj, int64_t* E) {
+    if (!ctx || !d_fw_beg || !d_fw_adj || !d_bw_beg || !d_bw_adj || !E) return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    return gen_rc(
+        gen_grid_device(ctx->stream, n, p_bidir, seed, d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj, E, ctx->err));
+}
+
+int ispan_build_csr_device(ispan_ctx* ctx, int32_t V, int64_t m, const int32_t* d_src, const int32_t* d_dst,
+                           int32_t** d_fw_beg, int32_t** d_fw_adj, int32_t** d_bw_beg, int32_t** d_bw_adj,
+                           int64_t* E) {
+    if (!ctx || (m > 0 && (!d_src || !d_dst)) || !d_fw_beg || !d_fw_adj || !d_bw_beg || !d_bw_adj || !E)
+        return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    return gen_rc(build_csr_device(ctx->stream, V, m, d_src, d_dst, d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj, E,
+                                   ctx->err));
+}
+
+void ispan_device_free(void* d_ptr) {
+    if (d_ptr) (void)hipFree(d_ptr);
+}
+
+}  // extern "C"
