@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X iSpan SCC pipeline.
+
+One step = one complete SCC solve (trim, giant FW/BW, trims, WCC, mice) of a
+device-resident graph, from label reset to canonical labels on the device.
+
+Default workload: one R-MAT graph, scale 24 (16,777,216 vertices), edge
+factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 1, self-loops and
+duplicates removed, int32 CSR in both directions -- generated on the GPU.
+
+Prints one JSON line.  `value` is input edges / solve time (edges/s per SCC
+solve).  `roofline` is the dominant kernel's algorithmic bytes per launch
+over its HIP-event-timed average launch duration, against 8 TB/s.
+`cpu_baseline` times the compiled reference (oracle/_ref, when present) and
+the oracle's OpenMP restatement on the host cores, on the same graph.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload rmat24]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "edges/s + HBM GB/s per SCC solve; batched graphs/s, p50/p99 latency vs 1/launch"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "rmat16": ("rmat", 16, 16.0),
+    "rmat20": ("rmat", 20, 16.0),
+    "rmat24": ("rmat", 24, 16.0),
+    "rmat26": ("rmat", 26, 16.0),
+    "grid4096": ("grid", 4096, 0.6),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_graph(S, workload: str, seed: int):
+    kind, size, param = WORKLOADS[workload]
+    if kind == "rmat":
+        return S.gen_rmat(1 << size, param, seed), {"workload": f"rmat-s{size}-ef{int(param)}", "scale": size,
+                                                    "edge_factor": param, "abcd": [0.57, 0.19, 0.19, 0.05],
+                                                    "seed": seed}
+    return S.gen_grid(size, param, seed), {"workload": f"grid-{size}x{size}", "n": size, "p_bidir": param,
+                                           "long_range_frac": 0.01, "seed": seed}
+
+
+def algorithmic_bytes(cls: int, kp, V: int) -> float:
+    """Per-class algorithmic bytes: 4 per adjacency entry read, 12 per vertex
+    reached; trim-1-first reads both offset arrays and writes label + queue."""
+    if cls == 3:
+        return 12.0 * V * kp.launches[3] + 4.0 * kp.found[3]
+    return 4.0 * kp.edges[cls] + 12.0 * kp.found[cls]
+
+
+def cpu_leg(dg, V: int, E: int, gpu_labels: np.ndarray, threads: int, runs: int, timeout: float) -> dict:
+    import oracle as O
+    t0 = time.time()
+    hg = dg.download()
+    g = O.Graph(hg.fw_beg, hg.fw_adj, hg.bw_beg, hg.bw_adj)
+    log(f"[cpu] graph on host in {time.time() - t0:.1f}s")
+    out = {}
+    # the oracle's OpenMP restatement ("port")
+    times = []
+    lab = None
+    for _ in range(runs):
+        lab, tm, _c = O.ispan(g, threads=threads)
+        times.append(tm["total"])
+    port_ok = bool(np.array_equal(lab, gpu_labels))
+    out["port"] = {"ms_mean": statistics.fmean(times), "ms_median": statistics.median(times), "runs": runs,
+                   "threads": threads, "labels_equal_gpu": port_ok}
+    log(f"[cpu] port: {out['port']}")
+    # the compiled reference, if it was built here and travelled with the tree
+    if O.have_ref():
+        try:
+            with tempfile.TemporaryDirectory(dir="/tmp") as d:
+                paths = O.write_ispan_files(g, d)
+                lab_path = os.path.join(d, "labels.bin")
+                env = dict(os.environ, ISPAN_REF_LABELS=lab_path, OMP_NUM_THREADS=str(threads))
+                r = subprocess.run([O.REF_BIN, *paths, str(threads), "30", "200", "10", "0.1", str(runs)], env=env,
+                                   capture_output=True, text=True, timeout=timeout)
+                if r.returncode == 0:
+                    per_run = [float(l.split(",")[1]) for l in r.stdout.splitlines() if l.startswith("total time,")]
+                    raw = np.fromfile(lab_path, dtype=np.int32)
+                    ref_ok = bool(np.array_equal(O.canon_ref_raw(g, raw), gpu_labels))
+                    out["reference"] = {"ms_mean": statistics.fmean(per_run), "ms_median": statistics.median(per_run),
+                                        "avg_total_ms": O.parse_ref_average(r.stdout).get("Total time"),
+                                        "runs": runs, "threads": threads, "labels_equal_gpu": ref_ok}
+                    log(f"[cpu] reference: {out['reference']}")
+                else:
+                    out["reference_error"] = f"exit {r.returncode}"
+        except subprocess.TimeoutExpired:
+            out["reference_error"] = "timeout"
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--timing", type=int, default=2, help="library timing level inside the timed region")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build runs the SCC pipeline on a single MI355X "
+              "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    import ispan_amd as I
+
+    torch.cuda.init()
+    S = I.Solver(0)
+    t0 = time.time()
+    dg, cfg = build_graph(S, a.workload, a.seed)
+    V, E = dg.V, dg.E
+    log(f"[gpu] {cfg['workload']}: V={V} E={E} built in {time.time() - t0:.2f}s")
+    S.bind(dg, I.default_params(validate=1))
+
+    p_fast = I.default_params(timing=0)
+    p_timed = I.default_params(timing=a.timing)
+    for _ in range(a.warmup):
+        S.solve(p_fast)
+    # per-phase breakdown (separate, untimed solve with phase events)
+    st_phase = S.solve(I.default_params(timing=1))
+    S.solve_stats(st_phase)
+
+    # ---- timed region: exactly `steps` solves
+    prof = [0.0] * 4
+    launches = [0] * 4
+    edges = [0] * 4
+    found = [0] * 4
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        S.solve(p_timed)
+        if a.timing >= 2:
+            kp = S.kernel_profile()
+            for c in range(4):
+                prof[c] += kp.ms[c]
+                launches[c] += kp.launches[c]
+                edges[c] += kp.edges[c]
+                found[c] += kp.found[c]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed * 1e3 / a.steps
+    labels = S.labels()
+
+    result = {
+        "metric": METRIC,
+        "value": E / (ms_per_step * 1e-3),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": dict(cfg, V=V, E=E),
+    }
+    # BFS phases against the HBM roofline (algorithmic bytes from the final sets)
+    fw_ms, bw_ms = st_phase.ms[7], st_phase.ms[8]
+    bfs_bytes = 4.0 * (st_phase.fw_edges + st_phase.bw_edges) + 12.0 * (st_phase.fw_size + st_phase.giant_size)
+    result["phases_ms"] = {k: round(v, 4) for k, v in st_phase.as_dict()["ms"].items()}
+    result["bfs"] = {"fw_ms": fw_ms, "bw_ms": bw_ms, "algorithmic_bytes": bfs_bytes,
+                     "gbps": bfs_bytes / ((fw_ms + bw_ms) * 1e-3) / 1e9 if fw_ms + bw_ms > 0 else None,
+                     "edges_per_s": (st_phase.fw_edges + st_phase.bw_edges) / ((fw_ms + bw_ms) * 1e-3)
+                     if fw_ms + bw_ms > 0 else None,
+                     "fw_size": st_phase.fw_size, "giant": st_phase.giant_size, "fw_edges": st_phase.fw_edges,
+                     "bw_edges": st_phase.bw_edges, "host_syncs": st_phase.host_syncs, "n_scc": st_phase.n_scc}
+    if a.timing >= 2 and sum(launches):
+        dom = max(range(4), key=lambda c: prof[c])
+        kp_tot = I.KernelProfile()
+        for c in range(4):
+            kp_tot.launches[c], kp_tot.ms[c], kp_tot.edges[c], kp_tot.found[c] = launches[c], prof[c], edges[c], \
+                found[c]
+        per_launch_bytes = algorithmic_bytes(dom, kp_tot, V) / launches[dom]
+        avg_ms = prof[dom] / launches[dom]
+        achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        result["roofline"] = {"bound": "hbm", "kernel": I.KernelProfile.CLASSES[dom], "achieved": achieved,
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                              "traffic": None, "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms,
+                              "launches": launches[dom]}
+        result["kernels"] = {I.KernelProfile.CLASSES[c]: {"launches": launches[c], "ms_total": prof[c],
+                                                          "gbps": algorithmic_bytes(c, kp_tot, V) / (prof[c] * 1e-3)
+                                                          / 1e9 if prof[c] > 0 else None}
+                             for c in range(4)}
+        pmc = os.path.join(ROOT, "profiles", f"pmc_{result['roofline']['kernel']}_{a.workload}.json")
+        if os.path.exists(pmc):
+            with open(pmc) as f:
+                d = json.load(f)
+            result["roofline"]["traffic"] = d.get("hbm_bytes_per_launch")
+            result["roofline"]["traffic_source"] = os.path.relpath(pmc, ROOT)
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        cpu = cpu_leg(dg, V, E, labels, threads, a.cpu_runs, timeout=600)
+        ref = cpu.get("reference")
+        base = ref if ref else cpu["port"]
+        result["cpu_baseline"] = {
+            "value": E / (base["ms_mean"] * 1e-3), "unit": "edges/s", "cores": threads,
+            "kind": "reference" if ref else "port",
+            "sample": f"full {cfg['workload']} graph, {a.cpu_runs} solves, mean of the reference's per-run "
+                      f"'total time' (wall clock around its parallel region)" if ref else
+                      f"full {cfg['workload']} graph, {a.cpu_runs} solves of the oracle restatement",
+            "ms_mean": base["ms_mean"], "ms_median": base["ms_median"],
+            "cpu_model": _cpu_model(), "detail": cpu}
+        result["gpu_over_cpu"] = base["ms_mean"] / ms_per_step
+        result["labels_equal_cpu"] = base["labels_equal_gpu"]
+    dg.free()
+    S.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -126,6 +126,20 @@ const int32_t* ispan_labels_device(const ispan_ctx* ctx);
 /* Fill stats->n_scc, fw_edges and bw_edges for the last solve (extra pass). */
 int ispan_solve_stats(ispan_ctx* ctx, ispan_stats* stats);
 
+/* Per-kernel profile of the last solve run with params.timing >= 2: every
+ * traversal launch is bracketed by HIP events on the solve stream.
+ * Classes: 0 top-down level (k_td), 1 top-down hub items (k_td_hub),
+ * 2 bottom-up sweep (k_bu), 3 trim-1-first + queue + pivot.
+ * edges = adjacency entries read, found = vertices reached; algorithmic
+ * bytes of a class = 4 * edges + 12 * found (trim-1-first: 12 V + 4 fq0). */
+typedef struct ispan_kernel_profile {
+    int64_t launches[4];
+    double ms[4];
+    int64_t edges[4];
+    int64_t found[4];
+} ispan_kernel_profile;
+int ispan_kernel_profile_get(const ispan_ctx* ctx, ispan_kernel_profile* out);
+
 /* One-shot host-pointer convenience: upload, solve, download (the whole of
  * scc_detection + the label hook of get_scc_result).  labels: int32[V]. */
 int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,

@@ -35,7 +35,7 @@ EXPORTS = [
     "ispan_ctx_last_error", "ispan_graph_upload", "ispan_graph_bind_device", "ispan_solve", "ispan_labels_download",
     "ispan_labels_device", "ispan_solve_stats", "ispan_scc_detection", "ispan_graph_load", "ispan_graph_free",
     "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
-    "ispan_build_csr_device", "ispan_device_free", "ispan_version",
+    "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
 ]
 
 
@@ -72,6 +72,17 @@ class Stats(C.Structure):
                  13: "trim_size_3", 14: "trim_1_sweeps"}
         d["ms"] = {n: self.ms[i] for i, n in names.items()}
         return d
+
+
+class KernelProfile(C.Structure):
+    """ispan_kernel_profile: classes 0 top-down, 1 hub items, 2 bottom-up, 3 trim-1-first."""
+    _fields_ = [("launches", C.c_int64 * 4), ("ms", C.c_double * 4), ("edges", C.c_int64 * 4),
+                ("found", C.c_int64 * 4)]
+    CLASSES = ("k_td", "k_td_hub", "k_bu", "k_trim1_first")
+
+    def as_dict(self) -> dict:
+        return {n: {"launches": self.launches[i], "ms": self.ms[i], "edges": self.edges[i], "found": self.found[i]}
+                for i, n in enumerate(self.CLASSES)}
 
 
 class HostGraphC(C.Structure):
@@ -129,6 +140,7 @@ def lib():
         "ispan_build_csr_device": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _i32pp, _i32pp, _i32pp, _i32pp,
                                              C.POINTER(C.c_int64)]),
         "ispan_device_free": (None, [_vp]),
+        "ispan_kernel_profile_get": (C.c_int, [_vp, C.POINTER(KernelProfile)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -299,6 +311,11 @@ class Solver:
     def solve_stats(self, st: Stats) -> Stats:
         _check(lib().ispan_solve_stats(self._ctx, C.byref(st)), self._ctx)
         return st
+
+    def kernel_profile(self) -> KernelProfile:
+        kp = KernelProfile()
+        _check(lib().ispan_kernel_profile_get(self._ctx, C.byref(kp)), self._ctx)
+        return kp
 
     def labels(self) -> np.ndarray:
         out = np.empty(max(self.V, 0), np.int32)

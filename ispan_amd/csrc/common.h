@@ -38,6 +38,8 @@ struct alignas(128) Ctl {
     int64_t fw_edges;       // statistics pass
     int64_t bw_edges;
     int64_t n_scc;
+    int64_t work_edges[3];  // adjacency entries read: 0 top-down, 1 hub items, 2 bottom-up
+    int64_t work_found[3];  // vertices reached, same classes
     int32_t pad[2];
 };
 
@@ -69,7 +71,8 @@ hipError_t launch_trim1_first(const int32_t* fwb, const int32_t* bwb, int32_t V,
 // One top-down level: expand q[par] into q[par^1] (forward, bw == false:
 // out-edges, claim act & ~fvis; backward: in-edges, claim fvis & ~bvis).
 hipError_t launch_td_level(bool bw, const int32_t* beg, const int32_t* adj, const uint64_t* allow, uint64_t* vis,
-                           int32_t* const q[2], HubItem* const hub[2], Ctl* ctl, int par, int grid, hipStream_t s);
+                           int32_t* const q[2], HubItem* const hub[2], Ctl* ctl, int par, int grid, hipStream_t s,
+                           hipEvent_t ev_mid = nullptr);
 // One bottom-up sweep over the candidates allow & ~vis; pushes every vertex
 // it visits into q[par^1].
 hipError_t launch_bu_level(bool bw, const int32_t* beg, const int32_t* adj, const uint64_t* allow, uint64_t* vis,

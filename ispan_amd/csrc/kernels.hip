@@ -114,9 +114,10 @@ __device__ __forceinline__ bool try_claim(int32_t w, const uint64_t* __restrict_
 constexpr int kLaneEdges = 32;
 template <class Pred>
 __device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t end, int32_t self,
-                                              const int32_t* __restrict__ adj, Pred pred) {
+                                              const int32_t* __restrict__ adj, Pred pred, int& scanned) {
     bool found = false;
     if (active) {
+        const int32_t p0 = p;
         const int32_t lim = min(end, p + kLaneEdges);
         while (p + 4 <= lim && !found) {
             int32_t a0 = adj[p], a1 = adj[p + 1], a2 = adj[p + 2], a3 = adj[p + 3];
@@ -124,13 +125,15 @@ __device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t en
             p += 4;
         }
         while (p < lim && !found) found = pred(adj[p++], self);
+        scanned += p - p0;
     }
     uint64_t pend = __ballot(active && !found && p < end);
     while (pend) {
         const int l = __ffsll((unsigned long long)pend) - 1;
         const int32_t pp = __shfl(p, l, kWave), ee = __shfl(end, l, kWave), ss = __shfl(self, l, kWave);
         bool hit = false;
-        for (int32_t qb = pp; qb < ee; qb += kWave) {
+        int32_t qb = pp;
+        for (; qb < ee; qb += kWave) {
             const int32_t q = qb + lane_id();
             const bool h = q < ee && pred(adj[q], ss);
             if (__ballot(h)) {
@@ -138,10 +141,20 @@ __device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t en
                 break;
             }
         }
-        if (lane_id() == l) found = hit;
+        if (lane_id() == l) {
+            found = hit;
+            scanned += min(qb + kWave, ee) - pp;
+        }
         pend &= pend - 1;
     }
     return found;
+}
+
+// Adds a per-thread count into a 64-bit global counter, one atomic per wave.
+__device__ __forceinline__ void wave_add64(int64_t* dst, int64_t x) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    if (lane_id() == 0 && x) atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)x);
 }
 
 __device__ __forceinline__ bool pivot_better(int64_t p, int32_t i, int64_t bp, int32_t bi) {
@@ -330,6 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_td(const int32_t* __restrict__ beg, 
     int32_t* hub_n = &ctl->hub_n[par];
     WaveQueue wq{s_wbuf[wave_id()], 0};
     int gmin = INT32_MAX;
+    int64_t edges = 0, found = 0;
     for (int64_t tb = (int64_t)blockIdx.x * kBlock; tb < n; tb += (int64_t)gridDim.x * kBlock) {
         const int64_t i = tb + tid;
         int32_t start = 0, deg = 0;
@@ -348,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_td(const int32_t* __restrict__ beg, 
         block_excl_scan(deg, excl, total, s_tmp);
         s_off[tid] = excl;
         s_start[tid] = start;
+        if (tid == 0) edges += total;
         __syncthreads();
         for (int eb = 0; eb < total; eb += kBlock) {
             const int e = eb + tid;
@@ -364,11 +379,14 @@ __global__ __launch_bounds__(kBlock) void k_td(const int32_t* __restrict__ beg, 
                 claim = try_claim(w, allow, vis);
             }
             wq_push(wq, claim, w, qout, qn_out);
+            found += claim;
             if (BW && claim) gmin = min(gmin, w);
         }
         __syncthreads();
     }
     wq_flush(wq, qout, qn_out);
+    wave_add64(&ctl->work_edges[0], edges);
+    wave_add64(&ctl->work_found[0], found);
     if (BW) {
         gmin = wave_min(gmin);
         if (lane_id() == 0 && gmin != INT32_MAX) atomicMin(&ctl->giant_min, gmin);
@@ -394,9 +412,11 @@ __global__ __launch_bounds__(kBlock) void k_td_hub(const int32_t* __restrict__ b
     int32_t* qn_out = &ctl->q_n[par ^ 1];
     WaveQueue wq{s_wbuf[wave_id()], 0};
     int gmin = INT32_MAX;
+    int64_t edges = 0, found = 0;
     for (int h = blockIdx.x; h < nh; h += gridDim.x) {
         const HubItem it = hub_in[h];
         const int32_t end = min(it.start + kHubChunk, beg[it.v + 1]);
+        if (tid == 0) edges += end - it.start;
         for (int32_t eb = it.start; eb < end; eb += kBlock) {
             const int32_t e = eb + tid;
             bool claim = false;
@@ -406,10 +426,13 @@ __global__ __launch_bounds__(kBlock) void k_td_hub(const int32_t* __restrict__ b
                 claim = try_claim(w, allow, vis);
             }
             wq_push(wq, claim, w, qout, qn_out);
+            found += claim;
             if (BW && claim) gmin = min(gmin, w);
         }
     }
     wq_flush(wq, qout, qn_out);
+    wave_add64(&ctl->work_edges[1], edges);
+    wave_add64(&ctl->work_found[1], found);
     if (BW) {
         gmin = wave_min(gmin);
         if (lane_id() == 0 && gmin != INT32_MAX) atomicMin(&ctl->giant_min, gmin);
@@ -438,6 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_bu(const int32_t* __restrict__ beg, 
     int32_t* qn_out = &ctl->q_n[par ^ 1];
     WaveQueue wq{s_wbuf[wave_id()], 0};
     int gmin = INT32_MAX;
+    int scanned = 0, nfound = 0;
     auto visited = [vis](int32_t u, int32_t) { return bit_of(vis, u); };
     for (int64_t t = blockIdx.x; t * kBuTileWords < nwords; t += gridDim.x) {
         if (wave_id() == 0) {
@@ -462,15 +486,18 @@ __global__ __launch_bounds__(kBlock) void k_bu(const int32_t* __restrict__ beg, 
             const int32_t v = have ? s_list[k] : 0;
             const int32_t p = have ? beg[v] : 0;
             const int32_t end = have ? beg[v + 1] : 0;
-            const bool found = wave_any_edge(have, p, end, v, adj, visited);
+            const bool found = wave_any_edge(have, p, end, v, adj, visited, scanned);
             if (found)
                 atomicOr(reinterpret_cast<unsigned long long*>(vis + (v >> 6)), 1ull << (v & 63));
             wq_push(wq, found, v, qout, qn_out);
+            nfound += found;
             if (BW && found) gmin = min(gmin, v);
         }
         __syncthreads();
     }
     wq_flush(wq, qout, qn_out);
+    wave_add64(&ctl->work_edges[2], scanned);
+    wave_add64(&ctl->work_found[2], nfound);
     if (BW) {
         gmin = wave_min(gmin);
         if (lane_id() == 0 && gmin != INT32_MAX) atomicMin(&ctl->giant_min, gmin);
@@ -545,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_trim1(const int32_t* __restrict__ fw
     const int n = ctl->fq_n[sel];
     int32_t* cnt = &ctl->fq_n[sel ^ 1];
     WaveQueue wq{s_wbuf[wave_id()], 0};
-    int trimmed = 0;
+    int trimmed = 0, scanned = 0;
     auto live = [act](int32_t w, int32_t self) { return w != self && bit_of(act, w); };
     for (int64_t tb = (int64_t)blockIdx.x * kBlock; tb < n; tb += (int64_t)gridDim.x * kBlock) {
         const int64_t i = tb + threadIdx.x;
@@ -555,8 +582,9 @@ __global__ __launch_bounds__(kBlock) void k_trim1(const int32_t* __restrict__ fw
             v = fq_in[i];
             alive = bit_of(act, v);
         }
-        const bool out = wave_any_edge(alive, alive ? fwb[v] : 0, alive ? fwb[v + 1] : 0, v, fwa, live);
-        const bool in = wave_any_edge(alive && out, alive ? bwb[v] : 0, alive ? bwb[v + 1] : 0, v, bwa, live);
+        const bool out = wave_any_edge(alive, alive ? fwb[v] : 0, alive ? fwb[v + 1] : 0, v, fwa, live, scanned);
+        const bool in =
+            wave_any_edge(alive && out, alive ? bwb[v] : 0, alive ? bwb[v + 1] : 0, v, bwa, live, scanned);
         const bool trim = alive && !(out && in);
         if (trim) {
             lab[v] = v;
@@ -947,18 +975,21 @@ hipError_t launch_trim1_first(const int32_t* fwb, const int32_t* bwb, int32_t V,
 }
 
 hipError_t launch_td_level(bool bw, const int32_t* beg, const int32_t* adj, const uint64_t* allow, uint64_t* vis,
-                           int32_t* const q[2], HubItem* const hub[2], Ctl* ctl, int par, int grid, hipStream_t s) {
-    if (bw) {
+                           int32_t* const q[2], HubItem* const hub[2], Ctl* ctl, int par, int grid, hipStream_t s,
+                           hipEvent_t ev_mid) {
+    if (bw)
         hipLaunchKernelGGL(k_td<true>, dim3(grid), dim3(kBlock), 0, s, beg, adj, allow, vis, q[par], q[par ^ 1],
                            hub[par], ctl, par);
-        hipLaunchKernelGGL(k_td_hub<true>, dim3(grid), dim3(kBlock), 0, s, beg, adj, allow, vis, hub[par],
-                           q[par ^ 1], ctl, par);
-    } else {
+    else
         hipLaunchKernelGGL(k_td<false>, dim3(grid), dim3(kBlock), 0, s, beg, adj, allow, vis, q[par], q[par ^ 1],
                            hub[par], ctl, par);
+    if (ev_mid) (void)hipEventRecord(ev_mid, s);
+    if (bw)
+        hipLaunchKernelGGL(k_td_hub<true>, dim3(grid), dim3(kBlock), 0, s, beg, adj, allow, vis, hub[par],
+                           q[par ^ 1], ctl, par);
+    else
         hipLaunchKernelGGL(k_td_hub<false>, dim3(grid), dim3(kBlock), 0, s, beg, adj, allow, vis, hub[par],
                            q[par ^ 1], ctl, par);
-    }
     return hipGetLastError();
 }
 

@@ -64,6 +64,15 @@ struct ispan_ctx {
     Ctl* h_ctl = nullptr;  // pinned
     hipEvent_t ev[16] = {};
     bool have_solution = false;
+    // kernel profiling (params.timing >= 2)
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    struct Rec {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    ispan_kernel_profile prof{};
 };
 
 namespace {
@@ -156,6 +165,37 @@ int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
     return ISPAN_OK;
 }
 
+hipEvent_t pool_event(ispan_ctx* ctx) {
+    if (ctx->pool_used == ctx->pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->pool.push_back(e);
+    }
+    return ctx->pool[ctx->pool_used++];
+}
+
+// Brackets of one traversal level when profiling: returns the events to
+// record before / between / after its kernels (null when not profiling).
+struct LevelEv {
+    hipEvent_t a = nullptr, mid = nullptr, b = nullptr;
+};
+
+LevelEv level_events(ispan_ctx* ctx, const ispan_params* P, bool td) {
+    LevelEv e;
+    if (P->timing < 2) return e;
+    e.a = pool_event(ctx);
+    e.b = pool_event(ctx);
+    if (td) e.mid = pool_event(ctx);
+    if (!e.a || !e.b || (td && !e.mid)) return LevelEv{};
+    if (td) {
+        ctx->recs.push_back({0, e.a, e.mid});
+        ctx->recs.push_back({1, e.mid, e.b});
+    } else {
+        ctx->recs.push_back({2, e.a, e.b});
+    }
+    return e;
+}
+
 int readback(ispan_ctx* ctx, int64_t* syncs) {
     CK(hipMemcpyAsync(ctx->h_ctl, ctx->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
@@ -186,7 +226,11 @@ int traverse(ispan_ctx* ctx, bool bw, int64_t set_size, const ispan_params* P, d
     while (true) {
         if (mode == 2) {
             for (int b = 0; b < batch; ++b) {
-                CK(launch_td_level(bw, td_beg, td_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->grid, s));
+                LevelEv le = level_events(ctx, P, true);
+                if (le.a) CK(hipEventRecord(le.a, s));
+                CK(launch_td_level(bw, td_beg, td_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->grid, s,
+                                   le.mid));
+                if (le.b) CK(hipEventRecord(le.b, s));
                 par ^= 1;
             }
             level += batch;
@@ -199,11 +243,15 @@ int traverse(ispan_ctx* ctx, bool bw, int64_t set_size, const ispan_params* P, d
             }
             continue;
         }
+        LevelEv le = level_events(ctx, P, mode == 0);
+        if (le.a) CK(hipEventRecord(le.a, s));
         if (mode == 0)
-            CK(launch_td_level(bw, td_beg, td_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->grid, s));
+            CK(launch_td_level(bw, td_beg, td_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->grid, s,
+                               le.mid));
         else
             CK(launch_bu_level(bw, bu_beg, bu_adj, allow, vis, ctx->q, ctx->hub, ctx->ctl, par, ctx->nwords,
                                ctx->grid, s));
+        if (le.b) CK(hipEventRecord(le.b, s));
         int rc = readback(ctx, syncs);
         if (rc) return rc;
         const int64_t frontier = ctx->h_ctl->q_n[par ^ 1];
@@ -265,6 +313,9 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     const double avg_degree = V > 0 ? (double)E / V : 0.0;
     const int G = ctx->grid;
 
+    ctx->recs.clear();
+    ctx->pool_used = 0;
+    ctx->prof = ispan_kernel_profile{};
     CK(launch_reset_ctl(ctx->ctl, s));
     rec(ctx, P, 0);
     if (V == 0) {
@@ -274,8 +325,14 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         return ISPAN_OK;
     }
     // P1-P3: trim-1-first, active queue, pivot
+    hipEvent_t t1a = nullptr, t1b = nullptr;
+    if (P->timing >= 2 && (t1a = pool_event(ctx)) && (t1b = pool_event(ctx))) {
+        ctx->recs.push_back({3, t1a, t1b});
+        CK(hipEventRecord(t1a, s));
+    }
     CK(launch_trim1_first(ctx->fwb, ctx->bwb, V, ctx->lab, ctx->act, ctx->fvis, ctx->bvis, ctx->fq[0], ctx->q[0],
                           ctx->ctl, ctx->part, P->pivot_wrap32, G, s));
+    if (t1b) CK(hipEventRecord(t1b, s));
     rec(ctx, P, 1);
     if ((rc = readback(ctx, &syncs))) return rc;
     const int64_t fq0 = ctx->h_ctl->fq_n[0];
@@ -421,6 +478,21 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         local.ms[1] = fw + bw;
         local.ms[2] = wcc + mice;
     }
+    if (P->timing >= 2) {
+        ispan_kernel_profile& pf = ctx->prof;
+        for (const auto& r : ctx->recs) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, r.a, r.b));
+            pf.launches[r.cls] += 1;
+            pf.ms[r.cls] += ms;
+        }
+        for (int c = 0; c < 3; ++c) {
+            pf.edges[c] = ctx->h_ctl->work_edges[c];
+            pf.found[c] = ctx->h_ctl->work_found[c];
+        }
+        pf.edges[3] = 0;
+        pf.found[3] = fq0;
+    }
     ctx->have_solution = true;
     if (st) *st = local;
     return ISPAN_OK;
@@ -518,6 +590,7 @@ int ispan_ctx_destroy(ispan_ctx* ctx) {
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->pool) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return ISPAN_OK;
@@ -743,6 +816,12 @@ int ispan_build_csr_device(ispan_ctx* ctx, int32_t V, int64_t m, const int32_t* 
     CK(hipSetDevice(ctx->device));
     return gen_rc(build_csr_device(ctx->stream, V, m, d_src, d_dst, d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj, E,
                                    ctx->err));
+}
+
+int ispan_kernel_profile_get(const ispan_ctx* ctx, ispan_kernel_profile* out) {
+    if (!ctx || !out) return ISPAN_ERR_ARG;
+    *out = ctx->prof;
+    return ISPAN_OK;
 }
 
 void ispan_device_free(void* d_ptr) {

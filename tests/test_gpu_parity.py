@@ -1,0 +1,259 @@
+"""GPU parity: canonical labels from the HIP pipeline (through the C ABI) must
+equal the oracle's -- Tarjan's labelling, the golden fixtures, and the
+sha256 of Tarjan's labels for the large seeded graphs.  Integer work: the
+bar is bit-exact equality."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import ispan_amd as I
+import oracle as O
+from tests import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def solve(solver, g, **kw):
+    solver.upload(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj))
+    st = solver.solve(I.default_params(**kw))
+    return solver.labels(), st
+
+
+def check(solver, g, **kw):
+    lab, st = solve(solver, g, **kw)
+    tj = O.tarjan(g)
+    bad = np.nonzero(lab != tj)[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first {bad[:8]}: gpu {lab[bad[:8]]} tarjan {tj[bad[:8]]}"
+    return st
+
+
+# ------------------------------------------------------------ fixtures
+
+def test_toy(solver):
+    g, d = G.toy_graph()
+    lab, st = solve(solver, g)
+    assert lab.tolist() == d["canonical"]
+    assert st.giant_size == 5 and st.trim2 == 2 and st.pivot == 1
+
+
+@pytest.mark.parametrize("name,d", G.pattern_cases(), ids=[n for n, _ in G.pattern_cases()])
+@pytest.mark.parametrize("wcc_mode", [0, 1])
+def test_pattern_graphs(solver, name, d, wcc_mode):
+    g = G.pattern_graph(d)
+    lab, st = solve(solver, g, wcc_mode=wcc_mode)
+    assert lab.tolist() == d["canonical"]
+    if name.startswith("pair"):
+        assert st.trim2 == 2
+    if name.startswith(("cycle3", "hub3")):
+        assert st.trim3 == 3
+
+
+# --------------------------------------------------------- random graphs
+
+PARAMS = [dict(), dict(wcc_mode=1), dict(pivot_wrap32=1), dict(alpha=1, beta=1, gamma=0),
+          dict(alpha=1000000, beta=2, gamma=1), dict(alpha=2, beta=1000000, gamma=100, async_batch=1),
+          dict(theta=0.9, trim_times=0), dict(theta=0.0, trim_times=5)]
+
+
+@pytest.mark.parametrize("seed", range(96))
+def test_random_graphs(solver, seed):
+    g = G.random_graph(seed, vmax=[50, 400, 3000][seed % 3], density=[0.6, 1.5, 4.0][(seed // 3) % 3])
+    check(solver, g, **PARAMS[seed % len(PARAMS)])
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_cycle_forests(solver, seed):
+    g = G.cycles_graph(seed, n_cycles=20 + 40 * seed)
+    st = check(solver, g, **PARAMS[seed % len(PARAMS)])
+    assert st.n_wcc >= 0
+
+
+@pytest.mark.parametrize("scale,ef,seed", [(8, 16, 1), (12, 4, 2), (14, 2, 3), (16, 16, 4), (18, 8, 5)])
+def test_rmat_vs_oracle_restatement(solver, scale, ef, seed):
+    g = O.rmat(scale, ef, seed)
+    lab, st = solve(solver, g)
+    olab, _t, oc = O.ispan(g, threads=4)
+    assert np.array_equal(lab, olab)
+    assert st.fq0 == oc["fq0"] and st.pivot == oc["pivot"] and st.giant_size == oc["giant_size"]
+    assert st.fw_size == oc["fw_size"]
+
+
+@pytest.mark.parametrize("n,p,seed", [(16, 0.6, 1), (64, 0.3, 2), (256, 0.6, 3), (256, 0.1, 4), (700, 0.5, 5)])
+def test_grids(solver, n, p, seed):
+    check(solver, O.grid(n, p, seed), wcc_mode=seed % 2)
+
+
+# ----------------------------------------------------------- edge cases
+
+def test_empty_and_edgeless(solver):
+    empty = O.Graph(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(1, np.int32), np.zeros(0, np.int32))
+    lab, st = solve(solver, empty)
+    assert lab.shape == (0,)
+    g = O.build_csr(9, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    lab, st = solve(solver, g)
+    assert lab.tolist() == list(range(9)) and st.pivot == -1
+
+
+def test_duplicates_self_loops_unsorted_rows(solver):
+    """The reference converter keeps duplicates, self-loops and input order
+    (graph_converter/txt_to_bin_fw_int.cpp:146-181); the pipeline must give
+    the same labels as on the cleaned graph."""
+    rng = np.random.default_rng(3)
+    V, m = 500, 2500
+    e = rng.integers(0, V, size=(m, 2)).astype(np.int32)
+    e = np.concatenate([e, e[:400], np.stack([np.arange(50), np.arange(50)], 1).astype(np.int32)])
+    order = np.argsort(e[:, 0], kind="stable")
+    src, dst = e[order, 0], e[order, 1]
+    fw_beg = np.zeros(V + 1, np.int32)
+    np.add.at(fw_beg, src + 1, 1)
+    fw_beg = np.cumsum(fw_beg).astype(np.int32)
+    o2 = np.argsort(dst, kind="stable")
+    bw_beg = np.zeros(V + 1, np.int32)
+    np.add.at(bw_beg, dst[o2] + 1, 1)
+    bw_beg = np.cumsum(bw_beg).astype(np.int32)
+    raw = O.Graph(fw_beg, dst.astype(np.int32), bw_beg, src[o2].astype(np.int32))
+    clean = O.build_csr(V, e[:, 0].copy(), e[:, 1].copy())
+    lab, _st = solve(solver, raw)
+    assert np.array_equal(lab, O.tarjan(clean))
+
+
+def test_repeat_and_regrow(solver):
+    big, small = O.rmat(16, 16, 9), O.rmat(10, 8, 9)
+    first, _ = solve(solver, big)
+    for _ in range(3):
+        st = solver.solve(I.default_params())
+        assert np.array_equal(solver.labels(), first)
+    lab_s, _ = solve(solver, small)
+    assert np.array_equal(lab_s, O.tarjan(small))
+    lab_b, _ = solve(solver, big)
+    assert np.array_equal(lab_b, first)
+
+
+def test_error_codes():
+    with I.Solver(0) as s:
+        with pytest.raises(I.IspanError) as e:
+            s.solve()
+        assert e.value.code == -8  # no graph bound
+        g, _d = G.toy_graph()
+        bad = g.fw_adj.copy()
+        bad[0] = 99
+        with pytest.raises(I.IspanError) as e:
+            s.upload(I.HostGraph(g.fw_beg, bad, g.bw_beg, g.bw_adj))
+        assert e.value.code == -2
+        s.upload(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj))
+        with pytest.raises(I.IspanError) as e:
+            s.solve(I.default_params(alpha=0))
+        assert e.value.code == -1
+        assert s.solve().giant_size == 5
+
+
+def test_one_shot_entry_point():
+    g = O.rmat(12, 16, 4)
+    lab = I.scc_detection(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj), 30, 200, 10, 0.1, 56)
+    assert np.array_equal(lab, O.tarjan(g))
+
+
+def test_labels_device_copy(solver):
+    import torch
+    g = O.rmat(12, 8, 6)
+    solver.upload(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj))
+    out = torch.full((g.V,), -7, dtype=torch.int32, device="cuda")
+    solver.solve(I.default_params(), d_labels=out.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), O.tarjan(g))
+    assert solver.labels_device_ptr() != 0
+
+
+def test_external_stream(solver):
+    import torch
+    g = O.rmat(13, 8, 8)
+    st = torch.cuda.Stream()
+    solver.set_stream(st.cuda_stream)
+    try:
+        lab, _ = solve(solver, g)
+    finally:
+        solver.set_stream(None)
+    assert np.array_equal(lab, O.tarjan(g))
+
+
+def test_stats_consistency(solver):
+    g = O.rmat(16, 16, 1)
+    lab, st = solve(solver, g, timing=2)
+    solver.solve_stats(st)
+    sizes = np.bincount(lab)
+    assert st.giant_size == sizes.max()
+    assert st.n_scc == np.count_nonzero(sizes)
+    assert st.fw_size >= st.giant_size
+    fw_edges_lb = int(np.diff(g.fw_beg)[lab == lab[st.pivot]].sum())
+    assert st.fw_edges >= fw_edges_lb and st.bw_edges > 0
+    kp = solver.kernel_profile()
+    assert kp.launches[3] == 1 and kp.found[3] == st.fq0
+    assert sum(kp.found[:3]) == st.fw_size + st.giant_size - 2  # every vertex but the two roots is claimed once
+
+
+# ------------------------------------------------ device graph producers
+
+@pytest.mark.parametrize("key,d", G.hash_cases(), ids=[k for k, _ in G.hash_cases()])
+def test_hashed_graphs_on_device(solver, key, d):
+    """Device generator == oracle generator (CSR sha256) and device labels ==
+    Tarjan's (label sha256), up to the full-size configurations."""
+    dg = solver.gen_rmat(1 << d["size"], float(d["param"]), d["seed"]) if d["kind"] == "rmat" else \
+        solver.gen_grid(d["size"], float(d["param"]), d["seed"])
+    try:
+        assert (dg.V, dg.E) == (d["V"], d["E"])
+        hg = dg.download()
+        assert G.sha(hg.fw_beg, hg.fw_adj, hg.bw_beg, hg.bw_adj) == d["csr_sha256"]
+        solver.bind(dg)
+        st = solver.solve(I.default_params())
+        lab = solver.labels()
+        assert G.sha(lab) == d["labels_sha256"]
+        assert st.giant_size == d["giant"]
+        v = np.arange(dg.V)
+        assert np.all(lab <= v) and np.all(lab[lab] == lab)  # canonical form
+        assert int(np.count_nonzero(lab == v)) == d["n_scc"]
+    finally:
+        dg.free()
+
+
+def test_build_csr_device_matches_oracle(solver):
+    import torch
+    rng = np.random.default_rng(12)
+    V, m = 5000, 40000
+    e = rng.integers(0, V, size=(m, 2)).astype(np.int32)
+    src = torch.from_numpy(e[:, 0].copy()).cuda()
+    dst = torch.from_numpy(e[:, 1].copy()).cuda()
+    dg = solver.build_csr(V, m, src.data_ptr(), dst.data_ptr())
+    try:
+        hg = dg.download()
+        og = O.build_csr(V, e[:, 0].copy(), e[:, 1].copy())
+        for a, b in zip((hg.fw_beg, hg.fw_adj, hg.bw_beg, hg.bw_adj), (og.fw_beg, og.fw_adj, og.bw_beg, og.bw_adj)):
+            assert np.array_equal(a, b)
+    finally:
+        dg.free()
+    dst[7] = V  # out of range
+    with pytest.raises(I.IspanError) as ex:
+        solver.build_csr(V, m, src.data_ptr(), dst.data_ptr())
+    assert ex.value.code == -2
+
+
+# ----------------------------------------------------------- drop-in CLI
+
+def test_cli_drop_in(tmp_path):
+    """The `ispan` executable takes the reference argv and prints its
+    grammar (src/scc_core.cpp:1066, src/load_print.cpp:58,69-90)."""
+    d = dict(G.hash_cases())["rmat-16-16-seed1"]
+    g = G.build_hashed(d)
+    paths = O.write_ispan_files(g, str(tmp_path))
+    out = tmp_path / "labels.bin"
+    r = subprocess.run([I.CLI_PATH, *paths, "56", "30", "200", "10", "0.1", "3"], capture_output=True, text=True,
+                       env=dict(os.environ, ISPAN_LABELS_OUT=str(out)), timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("args = 11\nThread = 56, alpha = 30, beta = 200, gamma = 10, theta = 0.1, run_times = 3")
+    assert r.stdout.count("\nRuntime: ") == 3 and r.stdout.count("\nResult:\n") == 3
+    assert f"largest, {d['giant']}" in r.stdout and f"total, {d['n_scc']}" in r.stdout
+    avg = O.parse_ref_average(r.stdout)
+    assert set(avg) >= {"Trim", "Elephant SCC", "Mice SCC", "Total time", "FW BFS", "BW BFS", "Wcc", "Mice fw-bw"}
+    assert G.sha(np.fromfile(out, np.int32)) == d["labels_sha256"]
