@@ -56,8 +56,11 @@ typedef struct ispan_params {
     int32_t timing;       /* 1: per-phase HIP event timing into ispan_stats.ms      */
     int32_t validate;     /* 1: check CSR structure when a graph is bound           */
     int32_t wcc_mode;     /* 0: union-find hooking, 1: label propagation + jumping  */
-    int32_t async_batch;  /* top-down levels enqueued per host check (default 8)    */
-    int32_t reserved[5];
+    int32_t async_batch;  /* engine 1: top-down levels enqueued per host check (8)  */
+    int32_t engine;       /* giant phase: 0 one cooperative persistent kernel with
+                             grid barriers (default), 1 one launch per level with
+                             host-side switch decisions                              */
+    int32_t reserved[4];
 } ispan_params;
 
 /* Per-solve statistics.  ms[] uses the reference's avg_time slot map
@@ -128,10 +131,12 @@ int ispan_solve_stats(ispan_ctx* ctx, ispan_stats* stats);
 
 /* Per-kernel profile of the last solve run with params.timing >= 2: every
  * traversal launch is bracketed by HIP events on the solve stream.
- * Classes: 0 top-down level (k_td), 1 top-down hub items (k_td_hub),
- * 2 bottom-up sweep (k_bu), 3 trim-1-first + queue + pivot.
- * edges = adjacency entries read, found = vertices reached; algorithmic
- * bytes of a class = 4 * edges + 12 * found (trim-1-first: 12 V + 4 fq0). */
+ * Engine 1 classes: 0 top-down level (k_td), 1 top-down hub items
+ * (k_td_hub), 2 bottom-up sweep (k_bu), 3 trim-1-first + queue + pivot.
+ * Engine 0: class 3 is the whole giant-phase kernel (k_giant: trim-1-first,
+ * pivot, both traversals, relabel); edges[0]/[2] and found[0]/[2] hold its
+ * top-down and bottom-up work.  edges = adjacency entries read, found =
+ * vertices reached. */
 typedef struct ispan_kernel_profile {
     int64_t launches[4];
     double ms[4];

@@ -54,7 +54,7 @@ class Params(C.Structure):
     _fields_ = [("alpha", C.c_int32), ("beta", C.c_int32), ("gamma", C.c_int32), ("theta", C.c_double),
                 ("trim_times", C.c_int32), ("thread_count", C.c_int32), ("pivot_wrap32", C.c_int32),
                 ("timing", C.c_int32), ("validate", C.c_int32), ("wcc_mode", C.c_int32),
-                ("async_batch", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("async_batch", C.c_int32), ("engine", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):
@@ -79,6 +79,7 @@ class KernelProfile(C.Structure):
     _fields_ = [("launches", C.c_int64 * 4), ("ms", C.c_double * 4), ("edges", C.c_int64 * 4),
                 ("found", C.c_int64 * 4)]
     CLASSES = ("k_td", "k_td_hub", "k_bu", "k_trim1_first")
+    CLASSES_COOP = ("giant_topdown", "-", "giant_bottomup", "k_giant")
 
     def as_dict(self) -> dict:
         return {n: {"launches": self.launches[i], "ms": self.ms[i], "edges": self.edges[i], "found": self.found[i]}

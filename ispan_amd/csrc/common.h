@@ -54,6 +54,63 @@ struct HubItem {
     int32_t start;  // first edge offset of this chunk
 };
 
+// ---- cooperative giant-phase kernel (giant.hip) --------------------------
+struct alignas(128) GLine {
+    int32_t v;
+    int32_t pad[31];
+};
+
+// Device state of one k_giant launch; zeroed before every launch.
+struct GiantState {
+    GLine sub[32];      // barrier arrival counters (two-level barrier)
+    GLine top;
+    GLine gen;          // barrier generation
+    GLine qn[2][3];     // [direction][level % 3] queue lengths
+    GLine hn[2][3];     // [direction][level % 3] hub work items
+    GLine err;          // 1 barrier timeout, 2 level cap, 4 queue count out of range
+    int64_t root, fq0, fw_count, giant, gmin, fq_n, fw_levels, bw_levels;
+    int64_t work[4];    // adjacency entries read top-down / bottom-up, vertices found top-down / bottom-up
+    int64_t ts[5];      // wall-clock stamps: start, pivot known, FW done, BW done, end
+    int64_t pad[3];
+};
+
+struct GiantArgs {
+    const int32_t* fwb;
+    const int32_t* fwa;
+    const int32_t* bwb;
+    const int32_t* bwa;
+    const int32_t* fw_first;  // first out-neighbour per vertex (-1: none)
+    const int32_t* bw_first;  // first in-neighbour per vertex
+    int32_t V;
+    int64_t nwords;
+    int32_t* lab;
+    uint64_t* act;
+    uint64_t* fvis;
+    uint64_t* bvis;
+    uint64_t* snap;
+    int32_t* q0;
+    int32_t* q1;
+    HubItem* h0;
+    HubItem* h1;
+    int32_t* fq;              // active queue after the giant phase
+    GiantState* gs;
+    int64_t* part;            // kGiantSlots x grid partial counts
+    int32_t* fq_n_out;        // receives the active queue length
+    int64_t hub_cap;          // capacity of h0 / h1
+    int32_t alpha, beta, gamma;
+    double avg_degree;
+    int32_t wrap32;
+};
+
+constexpr int kGiantSlots = 12;
+constexpr int kGiantHubDeg = 512;   // degree from which a claimed vertex is split into work items
+constexpr int kGiantChunk = 2048;   // edges per work item
+inline int64_t giant_hub_capacity(int64_t E) { return 5 * (E / kGiantChunk) + 64; }
+
+int giant_max_blocks_per_cu();
+hipError_t launch_giant(const GiantArgs& a, int grid, hipStream_t s);
+hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s);
+
 // Grid sizes for grid-stride kernels.
 struct Launch {
     int grid_v;     // sweeps over V

@@ -1,0 +1,554 @@
+// The giant-SCC phase as one cooperative, persistent kernel: trim-1-first and
+// the pivot (src/trim_1_gfq.h:9-33, src/scc_common.h:92-150), the forward
+// and backward direction-optimising traversals (src/fw_bw.h:396-698,
+// 944-1248) and the relabelling of what they found, with grid-wide barriers
+// between levels instead of host round trips.
+//
+// Every block runs the same control flow: after each barrier all blocks sum
+// the same per-block partial counts and take the same top-down/bottom-up
+// decision, so every block passes the same barriers.  Data written by other
+// blocks before a barrier is read with loads that bypass the per-CU cache.
+#include "common.h"
+#include "device_util.h"
+
+namespace ispan {
+using namespace dev;
+
+namespace {
+
+constexpr int kSpinCap = 1 << 24;  // barrier wait bound (~1 s); beyond it the kernel aborts
+constexpr int kBarGroups = 32;     // arrival counters of the two-level barrier
+constexpr int kW = kBlock / kWave;
+
+// Partial-count slots: part[slot * grid + block].
+enum Slot { kPivP, kPivI, kSurv, kFound0, kFound1, kMin0, kMin1, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
+static_assert(kFoundBU < kGiantSlots, "slot count");
+
+struct Smem {
+    int32_t off[kBlock];
+    int32_t start[kBlock];
+    int tmp[kW];
+    int64_t red[kW];
+    int64_t red2[kW];
+    int32_t bi[kW];
+    int ok;
+    int32_t wbuf[kW][kWaveBuf];
+};
+
+struct Ctx {
+    GiantArgs a;
+    int G, b, tid, lane, wave;
+    Smem& sm;
+    int64_t edges_td, edges_bu, found_td, found_bu;  // work counters (thread 0's are the block's)
+};
+
+// Two-level grid barrier: blocks arrive on one of 32 counters, the last of
+// each group arrives on the top counter, the last overall bumps `gen`.
+// Returns false (and flags an error) if a wait exceeds kSpinCap.
+__device__ bool gsync(GiantState* gs, int G, Smem& sm) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const int32_t g0 = load_agent(&gs->gen.v);
+        __threadfence();
+        const int grp = blockIdx.x % kBarGroups;
+        const int ngrp = min(kBarGroups, G);
+        const int gsize = (G - grp + kBarGroups - 1) / kBarGroups;
+        bool last = false;
+        if (atomicAdd(&gs->sub[grp].v, 1) == gsize - 1) {
+            __hip_atomic_store(&gs->sub[grp].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            if (atomicAdd(&gs->top.v, 1) == ngrp - 1) {
+                __hip_atomic_store(&gs->top.v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last = true;
+            }
+        }
+        if (last) {
+            __threadfence();
+            atomicAdd(&gs->gen.v, 1);
+        } else {
+            int spins = 0;
+            while (load_agent(&gs->gen.v) == g0) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kSpinCap) {
+                    atomicOr(&gs->err.v, 1);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        __threadfence();
+        sm.ok = ok && load_agent(&gs->err.v) == 0;
+    }
+    __syncthreads();
+    return sm.ok;
+}
+
+// Sum of a per-thread value over the block (every thread gets it).
+__device__ int64_t block_sum(int64_t x, Smem& sm) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    if (lane_id() == 0) sm.red[wave_id()] = x;
+    __syncthreads();
+    x = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3];
+    __syncthreads();
+    return x;
+}
+
+// Sum / minimum of one partial slot over all blocks (every block computes
+// the same value).
+__device__ int64_t sum_slot(const int64_t* part, int slot, int G, Smem& sm) {
+    int64_t x = 0;
+    for (int i = threadIdx.x; i < G; i += kBlock) x += load_agent(&part[(int64_t)slot * G + i]);
+    return block_sum(x, sm);
+}
+
+__device__ int64_t min_slot(const int64_t* part, int slot, int G, Smem& sm) {
+    int64_t x = INT64_MAX;
+    for (int i = threadIdx.x; i < G; i += kBlock) x = min(x, load_agent(&part[(int64_t)slot * G + i]));
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x = min(x, (int64_t)__shfl_xor(x, d, kWave));
+    if (lane_id() == 0) sm.red[wave_id()] = x;
+    __syncthreads();
+    x = min(min(sm.red[0], sm.red[1]), min(sm.red[2], sm.red[3]));
+    __syncthreads();
+    return x;
+}
+
+// Pushes a newly claimed vertex for expansion at the next level: vertices of
+// degree >= kGiantHubDeg become kGiantChunk-edge work items, others go to
+// the queue (degree-0 vertices have nothing to expand).  All lanes call it.
+__device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* __restrict__ beg, WaveQueue& wq,
+                                          int32_t* qout, int32_t* qn_out, HubItem* hout, int32_t* hn_out) {
+    int32_t d = 0, s0 = 0;
+    if (claim) {
+        s0 = beg[w];
+        d = beg[w + 1] - s0;
+        if (d >= kGiantHubDeg) {
+            const int nch = (d + kGiantChunk - 1) / kGiantChunk;
+            const int h = atomicAdd(hn_out, nch);
+            for (int c = 0; c < nch; ++c) hout[h + c] = HubItem{w, s0 + c * kGiantChunk};
+        }
+    }
+    wq_push(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
+}
+
+// One direction of the traversal.  Returns the number of vertices claimed
+// (the root excluded), or -1 after a failed barrier.  BW: claims are limited
+// to the forward set and *gmin receives the smallest id reached.
+template <bool BW>
+__device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* levels_out, int32_t* gmin_out) {
+    const GiantArgs& a = c.a;
+    Smem& sm = c.sm;
+    const int32_t* beg = BW ? a.bwb : a.fwb;  // expansion (top-down) direction
+    const int32_t* adj = BW ? a.bwa : a.fwa;
+    const int32_t* rbeg = BW ? a.fwb : a.bwb;  // bottom-up scans the reverse direction
+    const int32_t* radj = BW ? a.fwa : a.bwa;
+    const int32_t* rfirst = BW ? a.fw_first : a.bw_first;
+    const uint64_t* allow = BW ? a.fvis : a.act;
+    uint64_t* vis = BW ? a.bvis : a.fvis;
+    GiantState* gs = a.gs;
+    const int dir = BW ? 1 : 0;
+    WaveQueue wq{sm.wbuf[c.wave], 0};
+
+    int64_t visited = 0;
+    int32_t gmin = BW ? root : INT32_MAX;
+    int mode = 0;  // 0 top-down, 1 bottom-up, 2 top-down to exhaustion
+    int64_t L = 0;
+    const int64_t total_waves = (int64_t)c.G * kW;
+    const int64_t gwave = (int64_t)c.b * kW + c.wave;
+    auto is_vis = [vis](int32_t u, int32_t) { return bit_of(vis, u); };
+
+    while (true) {
+        const int32_t* qin = (L & 1) ? a.q1 : a.q0;
+        int32_t* qout = (L & 1) ? a.q0 : a.q1;
+        const HubItem* hin = (L & 1) ? a.h1 : a.h0;
+        HubItem* hout = (L & 1) ? a.h0 : a.h1;
+        int32_t* qn_out = &gs->qn[dir][(L + 1) % 3].v;
+        int32_t* hn_out = &gs->hn[dir][(L + 1) % 3].v;
+        if (c.b == 0 && c.tid == 0) {  // counters of level L+2: last read during level L-1
+            __hip_atomic_store(&gs->qn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gs->hn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const bool topdown = mode != 1;
+        int64_t my_found = 0;  // per thread
+        if (topdown) {
+            // ---- top-down: tiles of the queue, then hub work items; level 0
+            // expands the root's list in kGiantChunk-edge pieces.
+            int64_t n = 0, nh = 0;
+            if (L == 0) {
+                if (c.b == 0 && c.tid == 0)
+                    atomicOr(reinterpret_cast<unsigned long long*>(vis + (root >> 6)), 1ull << (root & 63));
+                nh = (beg[root + 1] - beg[root] + kGiantChunk - 1) / kGiantChunk;
+            } else {
+                n = load_agent(&gs->qn[dir][L % 3].v);
+                nh = load_agent(&gs->hn[dir][L % 3].v);
+                if (n < 0 || n > a.V || nh < 0 || nh > a.hub_cap) {  // cannot happen with exact claims
+                    if (c.tid == 0) atomicOr(&gs->err.v, 4);
+                    n = 0;
+                    nh = 0;
+                }
+            }
+            const int64_t ntiles = (n + kBlock - 1) / kBlock;
+            for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
+                if (u < ntiles) {
+                    const int64_t i = u * kBlock + c.tid;
+                    int32_t start = 0, deg = 0;
+                    if (i < n) {
+                        const int32_t v = load_agent(&qin[i]);
+                        start = beg[v];
+                        deg = beg[v + 1] - start;
+                    }
+                    int excl, total;
+                    block_excl_scan(deg, excl, total, sm.tmp);
+                    sm.off[c.tid] = excl;
+                    sm.start[c.tid] = start;
+                    if (c.tid == 0) c.edges_td += total;
+                    __syncthreads();
+                    for (int eb = 0; eb < total; eb += kBlock) {
+                        const int e = eb + c.tid;
+                        bool claim = false;
+                        int32_t w = 0;
+                        if (e < total) {
+                            int lo = 0, hi = kBlock - 1;  // last j with off[j] <= e
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (sm.off[mid] <= e) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            w = adj[sm.start[lo] + (e - sm.off[lo])];
+                            claim = w != root && try_claim(w, allow, vis);
+                        }
+                        my_found += claim;
+                        if (BW && claim) gmin = min(gmin, w);
+                        push_next(claim, w, beg, wq, qout, qn_out, hout, hn_out);
+                    }
+                    __syncthreads();
+                } else {
+                    int32_t v, s0;
+                    if (L == 0) {
+                        v = root;
+                        s0 = beg[root] + (int32_t)(u - ntiles) * kGiantChunk;
+                    } else {
+                        v = load_agent(&hin[u - ntiles].v);
+                        s0 = load_agent(&hin[u - ntiles].start);
+                    }
+                    const int32_t e1 = min(s0 + kGiantChunk, beg[v + 1]);
+                    if (c.tid == 0) c.edges_td += e1 - s0;
+                    for (int32_t eb = s0; eb < e1; eb += kBlock) {
+                        const int32_t e = eb + c.tid;
+                        bool claim = false;
+                        int32_t w = 0;
+                        if (e < e1) {
+                            w = adj[e];
+                            claim = w != root && try_claim(w, allow, vis);
+                        }
+                        my_found += claim;
+                        if (BW && claim) gmin = min(gmin, w);
+                        push_next(claim, w, beg, wq, qout, qn_out, hout, hn_out);
+                    }
+                }
+            }
+            wq_flush(wq, qout, qn_out);
+        } else {
+            // ---- bottom-up over the candidate bitmap, one wave per 64-vertex
+            // word; only that wave writes the word here.  Its visited bits
+            // before the sweep go to snap, so a later switch to top-down can
+            // gather this level's finds.
+            int scanned = 0;
+            for (int64_t w = gwave; w < a.nwords; w += total_waves) {
+                const uint64_t vw = load_agent(&vis[w]);
+                const uint64_t cand = load_agent(&allow[w]) & ~vw;
+                if (c.lane == 0) a.snap[w] = vw;
+                if (!cand) continue;
+                const int32_t v = (int32_t)(w * 64 + c.lane);
+                const bool is_c = (cand >> c.lane) & 1ull;
+                bool found = false;
+                int32_t p = 0, e = 0;
+                if (is_c) {
+                    const int32_t f = rfirst[v];
+                    found = f >= 0 && bit_of(vis, f);
+                    scanned += 1;
+                    if (!found) {
+                        p = rbeg[v] + 1;
+                        e = rbeg[v + 1];
+                    }
+                }
+                found = wave_any_edge(is_c && !found, p, e, v, radj, is_vis, scanned) || found;
+                const uint64_t fm = __ballot(found);
+                if (fm && c.lane == 0) vis[w] = vw | fm;
+                my_found += found;
+                if (BW && found) gmin = min(gmin, v);
+            }
+            c.edges_bu += block_sum(scanned, sm);
+        }
+        // ---- level totals: every block computes the same sums
+        const int64_t blk = block_sum(my_found, sm);
+        if (topdown) c.found_td += blk;
+        else c.found_bu += blk;
+        const int fslot = (L & 1) ? kFound1 : kFound0;
+        const int mslot = (L & 1) ? kMin1 : kMin0;
+        if (BW) {
+            const int32_t m = wave_min(gmin);
+            if (c.lane == 0) sm.tmp[c.wave] = m;
+            __syncthreads();
+        }
+        if (c.tid == 0) {
+            a.part[(int64_t)fslot * c.G + c.b] = blk;
+            if (BW) a.part[(int64_t)mslot * c.G + c.b] = min(min(sm.tmp[0], sm.tmp[1]), min(sm.tmp[2], sm.tmp[3]));
+        }
+        if (!gsync(gs, c.G, sm)) return -1;
+        const int64_t found = sum_slot(a.part, fslot, c.G, sm);
+        if (BW) *gmin_out = (int32_t)min_slot(a.part, mslot, c.G, sm);
+        visited += found;
+        ++L;
+        if (found == 0) break;
+        if (L > (int64_t)a.V + 2) {
+            if (c.tid == 0) atomicOr(&gs->err.v, 2);
+            return -1;
+        }
+        if (mode == 0) {
+            const double ef = (double)found * a.avg_degree;
+            const double er = (double)(set_size - visited) * a.avg_degree;
+            if (er / a.alpha < ef) mode = 1;
+        } else if (mode == 1 && ((set_size * 1.0 / a.beta) > found || L - 1 > a.gamma)) {
+            // gather this level's finds (vis & ~snap) as the next top-down input
+            int32_t* q_next = (L & 1) ? a.q1 : a.q0;
+            HubItem* h_next = (L & 1) ? a.h1 : a.h0;
+            int32_t* qn_next = &gs->qn[dir][L % 3].v;
+            int32_t* hn_next = &gs->hn[dir][L % 3].v;
+            for (int64_t w = gwave; w < a.nwords; w += total_waves) {
+                const uint64_t nw = load_agent(&vis[w]) & ~load_agent(&a.snap[w]);
+                const int32_t v = (int32_t)(w * 64 + c.lane);
+                push_next((nw >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, h_next, hn_next);
+            }
+            wq_flush(wq, q_next, qn_next);
+            mode = 2;
+            if (!gsync(gs, c.G, sm)) return -1;
+        }
+    }
+    *levels_out = L;
+    return visited;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
+    __shared__ Smem sm;
+    Ctx c{a, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, lane_id(), wave_id(), sm, 0, 0, 0, 0};
+    GiantState* gs = a.gs;
+    const int64_t total_waves = (int64_t)c.G * kW;
+    const int64_t gwave = (int64_t)c.b * kW + c.wave;
+    const bool stamp = c.b == 0 && c.tid == 0;  // phase timestamps (constant-rate clock)
+    if (stamp) gs->ts[0] = wall_clock64();
+
+    // ---- phase A: trim-1-first (raw degree 0 either way), act bitmap,
+    // cleared traversal bitmaps, pivot = largest out*in (lowest id on ties).
+    int64_t bestp = 0, surv = 0;
+    int32_t besti = INT32_MAX;
+    for (int64_t w = gwave; w < a.nwords; w += total_waves) {
+        const int64_t v = w * 64 + c.lane;
+        bool s = false;
+        if (v < a.V) {
+            const int32_t od = a.fwb[v + 1] - a.fwb[v];
+            const int32_t id = a.bwb[v + 1] - a.bwb[v];
+            s = od > 0 && id > 0;
+            if (s) {
+                const int64_t p = a.wrap32 ? (int64_t)(int32_t)((uint32_t)od * (uint32_t)id) : (int64_t)od * id;
+                if (p > 0 && pivot_better(p, (int32_t)v, bestp, besti)) {
+                    bestp = p;
+                    besti = (int32_t)v;
+                }
+            }
+        }
+        const uint64_t m = __ballot(s);
+        if (c.lane == 0) {
+            a.act[w] = m;
+            a.fvis[w] = 0;
+            a.bvis[w] = 0;
+            surv += __popcll(m);
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const int64_t op = __shfl_xor(bestp, d, kWave);
+        const int32_t oi = __shfl_xor(besti, d, kWave);
+        if (pivot_better(op, oi, bestp, besti)) {
+            bestp = op;
+            besti = oi;
+        }
+    }
+    if (c.lane == 0) {
+        sm.red2[c.wave] = bestp;
+        sm.bi[c.wave] = besti;
+    }
+    const int64_t surv_blk = block_sum(surv, sm);
+    if (c.tid == 0) {
+        for (int w = 1; w < kW; ++w)
+            if (pivot_better(sm.red2[w], sm.bi[w], bestp, besti)) {
+                bestp = sm.red2[w];
+                besti = sm.bi[w];
+            }
+        a.part[(int64_t)kPivP * c.G + c.b] = bestp;
+        a.part[(int64_t)kPivI * c.G + c.b] = besti;
+        a.part[(int64_t)kSurv * c.G + c.b] = surv_blk;
+    }
+    if (!gsync(gs, c.G, sm)) return;
+    // every block reduces the pivot candidates and the survivor count
+    bestp = 0;
+    besti = INT32_MAX;
+    for (int i = c.tid; i < c.G; i += kBlock) {
+        const int64_t p = load_agent(&a.part[(int64_t)kPivP * c.G + i]);
+        const int32_t id = (int32_t)load_agent(&a.part[(int64_t)kPivI * c.G + i]);
+        if (pivot_better(p, id, bestp, besti)) {
+            bestp = p;
+            besti = id;
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const int64_t op = __shfl_xor(bestp, d, kWave);
+        const int32_t oi = __shfl_xor(besti, d, kWave);
+        if (pivot_better(op, oi, bestp, besti)) {
+            bestp = op;
+            besti = oi;
+        }
+    }
+    if (c.lane == 0) {
+        sm.red2[c.wave] = bestp;
+        sm.bi[c.wave] = besti;
+    }
+    __syncthreads();
+    for (int w = 0; w < kW; ++w)
+        if (pivot_better(sm.red2[w], sm.bi[w], bestp, besti)) {
+            bestp = sm.red2[w];
+            besti = sm.bi[w];
+        }
+    __syncthreads();
+    const int64_t fq0 = sum_slot(a.part, kSurv, c.G, sm);
+    const int32_t root = (bestp > 0 && besti < a.V) ? besti : -1;
+    if (stamp) gs->ts[1] = wall_clock64();
+
+    int64_t fw_count = 0, giant = 0, fw_levels = 0, bw_levels = 0;
+    int32_t gmin = root;
+    if (root >= 0) {
+        const int64_t fwc = traverse<false>(c, root, fq0, &fw_levels, &gmin);
+        if (fwc < 0) return;
+        fw_count = fwc + 1;
+        if (stamp) gs->ts[2] = wall_clock64();
+        const int64_t bwc = traverse<true>(c, root, fw_count, &bw_levels, &gmin);
+        if (bwc < 0) return;
+        giant = bwc + 1;
+    }
+    if (stamp) gs->ts[3] = wall_clock64();
+
+    // ---- post: giant labels, act minus giant, every label written once;
+    // each block owns a contiguous word range so the rebuilt active queue
+    // comes out ascending without atomics.
+    const int64_t per = (a.nwords + c.G - 1) / c.G;
+    const int64_t wb = min((int64_t)c.b * per, a.nwords), we = min(wb + per, a.nwords);
+    int64_t cnt = 0;
+    for (int64_t w = wb + c.wave; w < we; w += kW) {
+        const uint64_t a0 = load_agent(&a.act[w]);
+        const uint64_t g = root >= 0 ? load_agent(&a.bvis[w]) : 0ull;
+        const int64_t v = w * 64 + c.lane;
+        if (v < a.V) {
+            const bool gi = (g >> c.lane) & 1ull, ac = (a0 >> c.lane) & 1ull;
+            a.lab[v] = gi ? gmin : (ac ? -1 : (int32_t)v);
+        }
+        if (c.lane == 0) {
+            if (g) a.act[w] = a0 & ~g;
+            cnt += __popcll(a0 & ~g);
+        }
+    }
+    const int64_t cnt_blk = block_sum(cnt, sm);
+    if (c.tid == 0) {
+        a.part[(int64_t)kPost * c.G + c.b] = cnt_blk;
+        a.part[(int64_t)kEdgeTD * c.G + c.b] = c.edges_td;
+        a.part[(int64_t)kEdgeBU * c.G + c.b] = c.edges_bu;
+        a.part[(int64_t)kFoundTD * c.G + c.b] = c.found_td;
+        a.part[(int64_t)kFoundBU * c.G + c.b] = c.found_bu;
+    }
+    if (!gsync(gs, c.G, sm)) return;
+    // exclusive prefix of the block counts, then an ordered block-local scan
+    int64_t off, tot;
+    {
+        int64_t x0 = 0, xt = 0;
+        for (int i = c.tid; i < c.G; i += kBlock) {
+            const int64_t x = load_agent(&a.part[(int64_t)kPost * c.G + i]);
+            xt += x;
+            if (i < c.b) x0 += x;
+        }
+        off = block_sum(x0, sm);
+        tot = block_sum(xt, sm);
+    }
+    for (int64_t base = wb; base < we; base += kBlock) {
+        const int64_t w = base + c.tid;
+        const uint64_t m = w < we ? a.act[w] : 0ull;
+        int excl, total;
+        block_excl_scan(__popcll(m), excl, total, sm.tmp);
+        uint64_t mm = m;
+        int64_t o = off + excl;
+        while (mm) {
+            const int bit = __ffsll((unsigned long long)mm) - 1;
+            a.fq[o++] = (int32_t)(w * 64 + bit);
+            mm &= mm - 1;
+        }
+        off += total;
+    }
+    if (c.b == 0) {
+        const int64_t w0 = sum_slot(a.part, kEdgeTD, c.G, sm), w1 = sum_slot(a.part, kEdgeBU, c.G, sm);
+        const int64_t w2 = sum_slot(a.part, kFoundTD, c.G, sm), w3 = sum_slot(a.part, kFoundBU, c.G, sm);
+        if (c.tid == 0) {
+            gs->root = root;
+            gs->fq0 = fq0;
+            gs->fw_count = fw_count;
+            gs->giant = giant;
+            gs->gmin = gmin;
+            gs->fq_n = tot;
+            gs->fw_levels = fw_levels;
+            gs->bw_levels = bw_levels;
+            gs->work[0] = w0;
+            gs->work[1] = w1;
+            gs->work[2] = w2;
+            gs->work[3] = w3;
+            *a.fq_n_out = (int32_t)tot;
+            gs->ts[4] = wall_clock64();
+        }
+    }
+}
+
+__global__ void k_zero_i64(int64_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// first[v] = the first entry of v's list (-1 if empty): the bottom-up sweep's
+// first probe reads 4 dense bytes instead of a cache line of the list.
+__global__ void k_first(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj, int32_t V, int32_t* first) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x)
+        first[v] = beg[v + 1] > beg[v] ? adj[beg[v]] : -1;
+}
+
+int giant_max_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kBlock, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+hipError_t launch_giant(const GiantArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<int64_t*>(a.gs),
+                       (int64_t)(sizeof(GiantState) / 8));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    GiantArgs copy = a;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kBlock), args, 0, s);
+}
+
+hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_first, dim3(grid), dim3(256), 0, s, beg, adj, V, first);
+    return hipGetLastError();
+}
+
+}  // namespace ispan

@@ -8,160 +8,12 @@
 // The bitmaps are 2 MB each at V = 2^24, so the random probes of the
 // traversals stay in L2/MALL while adjacency is streamed from HBM.
 #include "common.h"
+#include "device_util.h"
 
 namespace ispan {
 
-namespace {
+using namespace dev;
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
-__device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
-__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
-
-__device__ __forceinline__ bool bit_of(const uint64_t* bm, int32_t v) {
-    return (bm[v >> 6] >> (v & 63)) & 1ull;
-}
-
-// Load that bypasses the non-coherent per-CU cache (reads what other CUs'
-// atomics wrote).
-template <class T>
-__device__ __forceinline__ T load_agent(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Lanes of one wave exchange data through memory: order and publish it.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-__device__ __forceinline__ int wave_incl_scan(int x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        int y = __shfl_up(x, d, kWave);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-
-__device__ __forceinline__ int wave_sum(int x) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    return x;
-}
-
-__device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x = min(x, __shfl_xor(x, d, kWave));
-    return x;
-}
-
-// Block-wide exclusive scan (256 threads); every thread gets the total.
-__device__ __forceinline__ void block_excl_scan(int x, int& excl, int& total, int* s_tmp /*[4]*/) {
-    int incl = wave_incl_scan(x);
-    if (lane_id() == kWave - 1) s_tmp[wave_id()] = incl;
-    __syncthreads();
-    int w0 = s_tmp[0], w1 = s_tmp[1], w2 = s_tmp[2], w3 = s_tmp[3];
-    int wb = wave_id() == 0 ? 0 : wave_id() == 1 ? w0 : wave_id() == 2 ? w0 + w1 : w0 + w1 + w2;
-    excl = wb + incl - x;
-    total = w0 + w1 + w2 + w3;
-    __syncthreads();
-}
-
-// Per-wave staging of queue pushes in LDS: one global atomic per flush
-// instead of one per wave-step.  All 64 lanes must call push/flush together.
-struct WaveQueue {
-    int32_t* buf;  // kWaveBuf entries of LDS owned by this wave
-    int n;         // wave-uniform fill
-};
-
-__device__ __forceinline__ void wq_flush(WaveQueue& wq, int32_t* gq, int32_t* gcount) {
-    if (wq.n == 0) return;
-    wave_sync();
-    int base = 0;
-    if (lane_id() == 0) base = atomicAdd(gcount, wq.n);
-    base = __shfl(base, 0, kWave);
-    for (int i = lane_id(); i < wq.n; i += kWave) gq[base + i] = wq.buf[i];
-    wq.n = 0;
-    wave_sync();
-}
-
-__device__ __forceinline__ void wq_push(WaveQueue& wq, bool pred, int32_t v, int32_t* gq, int32_t* gcount) {
-    uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    int c = __popcll(m);
-    if (wq.n + c > kWaveBuf) wq_flush(wq, gq, gcount);
-    if (pred) wq.buf[wq.n + __popcll(m & lanemask_lt())] = v;
-    wq.n += c;
-}
-
-// Claim w for a traversal: it must be allowed (act for FW, fvis for BW) and
-// not yet visited; the atomic makes every vertex claimed exactly once.
-__device__ __forceinline__ bool try_claim(int32_t w, const uint64_t* __restrict__ allow, uint64_t* vis) {
-    const int64_t wi = w >> 6;
-    const uint64_t bit = 1ull << (w & 63);
-    if (!(allow[wi] & bit)) return false;
-    if (vis[wi] & bit) return false;
-    unsigned long long old = atomicOr(reinterpret_cast<unsigned long long*>(vis + wi), (unsigned long long)bit);
-    return !(old & bit);
-}
-
-// Does any edge in adj[p, end) satisfy pred(w, self)?  Each lane first walks
-// its own list for up to kLaneEdges edges; the lists still open after that
-// are finished by the whole wave, 64 edges per step, one list at a time.
-// Must be called by all 64 lanes.
-constexpr int kLaneEdges = 32;
-template <class Pred>
-__device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t end, int32_t self,
-                                              const int32_t* __restrict__ adj, Pred pred, int& scanned) {
-    bool found = false;
-    if (active) {
-        const int32_t p0 = p;
-        const int32_t lim = min(end, p + kLaneEdges);
-        while (p + 4 <= lim && !found) {
-            int32_t a0 = adj[p], a1 = adj[p + 1], a2 = adj[p + 2], a3 = adj[p + 3];
-            found = pred(a0, self) || pred(a1, self) || pred(a2, self) || pred(a3, self);
-            p += 4;
-        }
-        while (p < lim && !found) found = pred(adj[p++], self);
-        scanned += p - p0;
-    }
-    uint64_t pend = __ballot(active && !found && p < end);
-    while (pend) {
-        const int l = __ffsll((unsigned long long)pend) - 1;
-        const int32_t pp = __shfl(p, l, kWave), ee = __shfl(end, l, kWave), ss = __shfl(self, l, kWave);
-        bool hit = false;
-        int32_t qb = pp;
-        for (; qb < ee; qb += kWave) {
-            const int32_t q = qb + lane_id();
-            const bool h = q < ee && pred(adj[q], ss);
-            if (__ballot(h)) {
-                hit = true;
-                break;
-            }
-        }
-        if (lane_id() == l) {
-            found = hit;
-            scanned += min(qb + kWave, ee) - pp;
-        }
-        pend &= pend - 1;
-    }
-    return found;
-}
-
-// Adds a per-thread count into a 64-bit global counter, one atomic per wave.
-__device__ __forceinline__ void wave_add64(int64_t* dst, int64_t x) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    if (lane_id() == 0 && x) atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)x);
-}
-
-__device__ __forceinline__ bool pivot_better(int64_t p, int32_t i, int64_t bp, int32_t bi) {
-    return p > bp || (p == bp && i < bi);
-}
-
-}  // namespace
 
 // ------------------------------------------------------------------ reset
 

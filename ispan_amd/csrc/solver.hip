@@ -40,6 +40,8 @@ struct ispan_ctx {
     const int32_t* bwa = nullptr;
     int32_t* owned[4] = {nullptr, nullptr, nullptr, nullptr};
     int64_t nwords = 0;
+    int32_t* fw_first = nullptr;  // first out-/in-neighbour per vertex, built at bind
+    int32_t* bw_first = nullptr;
 
     // solve state (capacity cap_V / cap_E)
     int32_t cap_V = -1;
@@ -62,6 +64,14 @@ struct ispan_ctx {
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     Ctl* h_ctl = nullptr;  // pinned
+    // cooperative giant-phase kernel
+    int coop_grid = 0;
+    double clock_khz = 0.0;
+    uint64_t* snap = nullptr;
+    HubItem* gh[2] = {nullptr, nullptr};
+    int64_t* gpart = nullptr;
+    GiantState* gstate = nullptr;
+    GiantState* h_gstate = nullptr;  // pinned
     hipEvent_t ev[16] = {};
     bool have_solution = false;
     // kernel profiling (params.timing >= 2)
@@ -117,6 +127,10 @@ void free_solve_buffers(ispan_ctx* ctx) {
     dfree(ctx->starts);
     dfree(ctx->qbuf);
     dfree(ctx->part);
+    dfree(ctx->snap);
+    dfree(ctx->gh[0]);
+    dfree(ctx->gh[1]);
+    dfree(ctx->gpart);
     if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
     ctx->sort_tmp = nullptr;
     ctx->sort_tmp_bytes = 0;
@@ -127,6 +141,8 @@ void free_solve_buffers(ispan_ctx* ctx) {
 
 void free_graph(ispan_ctx* ctx) {
     for (auto& p : ctx->owned) dfree(p);
+    dfree(ctx->fw_first);
+    dfree(ctx->bw_first);
     ctx->fwb = ctx->fwa = ctx->bwb = ctx->bwa = nullptr;
     ctx->V = -1;
     ctx->E = 0;
@@ -156,6 +172,9 @@ int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
     CK(hipMalloc(&ctx->starts, nv * 4));
     CK(hipMalloc(&ctx->qbuf, nv * 4));
     CK(hipMalloc(&ctx->part, (size_t)ctx->grid * sizeof(PivotPart)));
+    CK(hipMalloc(&ctx->snap, nw * 8));
+    for (int i = 0; i < 2; ++i) CK(hipMalloc(&ctx->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem)));
+    CK(hipMalloc(&ctx->gpart, (size_t)kGiantSlots * std::max(ctx->coop_grid, 1) * 8));
     size_t bytes = 0;
     CK(rocprim::radix_sort_keys(nullptr, bytes, ctx->keys, ctx->keys_alt, (size_t)nv, 0, 64, ctx->stream));
     CK(hipMalloc(&ctx->sort_tmp, bytes + 256));
@@ -324,45 +343,123 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         if (st) *st = local;
         return ISPAN_OK;
     }
-    // P1-P3: trim-1-first, active queue, pivot
-    hipEvent_t t1a = nullptr, t1b = nullptr;
-    if (P->timing >= 2 && (t1a = pool_event(ctx)) && (t1b = pool_event(ctx))) {
-        ctx->recs.push_back({3, t1a, t1b});
-        CK(hipEventRecord(t1a, s));
-    }
-    CK(launch_trim1_first(ctx->fwb, ctx->bwb, V, ctx->lab, ctx->act, ctx->fvis, ctx->bvis, ctx->fq[0], ctx->q[0],
-                          ctx->ctl, ctx->part, P->pivot_wrap32, G, s));
-    if (t1b) CK(hipEventRecord(t1b, s));
-    rec(ctx, P, 1);
-    if ((rc = readback(ctx, &syncs))) return rc;
-    const int64_t fq0 = ctx->h_ctl->fq_n[0];
-    const int32_t root = ctx->h_ctl->root;
-    local.fq0 = fq0;
-    local.pivot = root;
-    rec(ctx, P, 2);
-    int64_t fw_count = 0;
-    if (root >= 0) {
-        // P4 forward traversal
-        if ((rc = traverse(ctx, false, fq0, P, avg_degree, &syncs))) return rc;
-        local.fw_levels = ctx->h_ctl->levels;
-        CK(launch_popcount(ctx->fvis, ctx->nwords, &ctx->ctl->fw_count, G, s));
+    int64_t fq0 = 0, n = 0, fw_count = 0;
+    int32_t root = -1;
+    double giant_phase_ms[3] = {0, 0, 0};  // engine 0: trim-1-first+pivot, FW, BW+relabel
+    if (P->engine == 0) {
+        // P1-P6 + relabel in one cooperative kernel
+        if (ctx->coop_grid <= 0) {
+            ctx->err = "cooperative launch is not available on this device";
+            return ISPAN_ERR_HIP;
+        }
+        GiantArgs ga{};
+        ga.fwb = ctx->fwb;
+        ga.fwa = ctx->fwa;
+        ga.bwb = ctx->bwb;
+        ga.bwa = ctx->bwa;
+        ga.fw_first = ctx->fw_first;
+        ga.bw_first = ctx->bw_first;
+        ga.V = V;
+        ga.nwords = ctx->nwords;
+        ga.lab = ctx->lab;
+        ga.act = ctx->act;
+        ga.fvis = ctx->fvis;
+        ga.bvis = ctx->bvis;
+        ga.snap = ctx->snap;
+        ga.q0 = ctx->q[0];
+        ga.q1 = ctx->q[1];
+        ga.h0 = ctx->gh[0];
+        ga.h1 = ctx->gh[1];
+        ga.fq = ctx->fq[1];
+        ga.gs = ctx->gstate;
+        ga.part = ctx->gpart;
+        ga.fq_n_out = &ctx->ctl->fq_n[1];
+        ga.hub_cap = giant_hub_capacity(E);
+        ga.alpha = P->alpha;
+        ga.beta = P->beta;
+        ga.gamma = P->gamma;
+        ga.avg_degree = avg_degree;
+        ga.wrap32 = P->pivot_wrap32;
+        hipEvent_t ga0 = nullptr, ga1 = nullptr;
+        if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) {
+            ctx->recs.push_back({3, ga0, ga1});
+            CK(hipEventRecord(ga0, s));
+        }
+        CK(launch_giant(ga, ctx->coop_grid, s));
+        if (ga1) CK(hipEventRecord(ga1, s));
+        rec(ctx, P, 1);
+        rec(ctx, P, 2);
         rec(ctx, P, 3);
+        rec(ctx, P, 4);
+        CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, sizeof(GiantState), hipMemcpyDeviceToHost, s));
         if ((rc = readback(ctx, &syncs))) return rc;
-        fw_count = ctx->h_ctl->fw_count;
-        // P5-P6 backward traversal inside the FW set
-        CK(launch_bw_init(ctx->bvis, ctx->q[0], ctx->ctl, s));
-        if ((rc = traverse(ctx, true, fw_count, P, avg_degree, &syncs))) return rc;
-        local.bw_levels = ctx->h_ctl->levels;
+        const GiantState& gs = *ctx->h_gstate;
+        if (gs.err.v) {
+            ctx->err = gs.err.v & 1   ? "giant phase: grid barrier timed out"
+                       : gs.err.v & 2 ? "giant phase: level cap exceeded"
+                                      : "giant phase: queue count out of range";
+            return gs.err.v & 2 ? ISPAN_ERR_ITER : ISPAN_ERR_HIP;
+        }
+        fq0 = gs.fq0;
+        root = (int32_t)gs.root;
+        fw_count = gs.fw_count;
+        n = gs.fq_n;
+        local.fq0 = fq0;
+        local.pivot = root;
+        local.fw_size = fw_count;
+        local.giant_size = gs.giant;
+        local.fw_levels = gs.fw_levels;
+        local.bw_levels = gs.bw_levels;
+        if (ctx->clock_khz > 0) {
+            const double k = 1.0 / ctx->clock_khz;  // ms per tick
+            giant_phase_ms[0] = (gs.ts[1] - gs.ts[0]) * k;
+            giant_phase_ms[1] = root >= 0 ? (gs.ts[2] - gs.ts[1]) * k : 0.0;
+            giant_phase_ms[2] = (gs.ts[4] - (root >= 0 ? gs.ts[2] : gs.ts[1])) * k;
+        }
+        ctx->prof.edges[0] = gs.work[0];
+        ctx->prof.edges[2] = gs.work[1];
+        ctx->prof.found[0] = gs.work[2];
+        ctx->prof.found[2] = gs.work[3];
     } else {
-        rec(ctx, P, 3);
+        // P1-P3: trim-1-first, active queue, pivot
+        hipEvent_t t1a = nullptr, t1b = nullptr;
+        if (P->timing >= 2 && (t1a = pool_event(ctx)) && (t1b = pool_event(ctx))) {
+            ctx->recs.push_back({3, t1a, t1b});
+            CK(hipEventRecord(t1a, s));
+        }
+        CK(launch_trim1_first(ctx->fwb, ctx->bwb, V, ctx->lab, ctx->act, ctx->fvis, ctx->bvis, ctx->fq[0],
+                              ctx->q[0], ctx->ctl, ctx->part, P->pivot_wrap32, G, s));
+        if (t1b) CK(hipEventRecord(t1b, s));
+        rec(ctx, P, 1);
+        if ((rc = readback(ctx, &syncs))) return rc;
+        fq0 = ctx->h_ctl->fq_n[0];
+        root = ctx->h_ctl->root;
+        local.fq0 = fq0;
+        local.pivot = root;
+        rec(ctx, P, 2);
+        if (root >= 0) {
+            // P4 forward traversal
+            if ((rc = traverse(ctx, false, fq0, P, avg_degree, &syncs))) return rc;
+            local.fw_levels = ctx->h_ctl->levels;
+            CK(launch_popcount(ctx->fvis, ctx->nwords, &ctx->ctl->fw_count, G, s));
+            rec(ctx, P, 3);
+            if ((rc = readback(ctx, &syncs))) return rc;
+            fw_count = ctx->h_ctl->fw_count;
+            // P5-P6 backward traversal inside the FW set
+            CK(launch_bw_init(ctx->bvis, ctx->q[0], ctx->ctl, s));
+            if ((rc = traverse(ctx, true, fw_count, P, avg_degree, &syncs))) return rc;
+            local.bw_levels = ctx->h_ctl->levels;
+        } else {
+            rec(ctx, P, 3);
+        }
+        local.fw_size = fw_count;
+        CK(launch_post_giant(V, ctx->lab, ctx->act, ctx->bvis, ctx->fq[1], ctx->ctl, G, s));
+        rec(ctx, P, 4);
+        if ((rc = readback(ctx, &syncs))) return rc;
+        n = ctx->h_ctl->fq_n[1];
+        local.giant_size = root >= 0 ? fq0 - n : 0;  // giant = removed by post_giant
     }
-    local.fw_size = fw_count;
-    CK(launch_post_giant(V, ctx->lab, ctx->act, ctx->bvis, ctx->fq[1], ctx->ctl, G, s));
-    rec(ctx, P, 4);
-    if ((rc = readback(ctx, &syncs))) return rc;
     int sel = 1;
-    int64_t n = ctx->h_ctl->fq_n[1];
-    local.giant_size = root >= 0 ? fq0 - n : 0;  // giant = removed by post_giant
     int64_t trim1 = 0;
 
     // P7 trim-1 sweeps with the theta stop rule (src/scc_core.cpp:562-602):
@@ -461,7 +558,13 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         // event map: 0 start, 1 trim-1-first+queue+pivot, 2 pivot readback,
         // 3 FW done, 4 BW + giant relabel done, 5 trim-1 sweeps, 6 size-2/3,
         // 7 last trim-1 sweep, 8 WCC, 9 mice
-        const double first = ev_ms(ctx, 0, 1), fw = ev_ms(ctx, 2, 3), bw = ev_ms(ctx, 3, 4),
+        double first = ev_ms(ctx, 0, 1), fw = ev_ms(ctx, 2, 3), bw = ev_ms(ctx, 3, 4);
+        if (P->engine == 0) {
+            first = giant_phase_ms[0];
+            fw = giant_phase_ms[1];
+            bw = giant_phase_ms[2];
+        }
+        const double
                      t1 = ev_ms(ctx, 4, 5), t23 = ev_ms(ctx, 5, 6), t1b = ev_ms(ctx, 6, 7), wcc = ev_ms(ctx, 7, 8),
                      mice = ev_ms(ctx, 8, 9);
         local.ms[11] = first;
@@ -486,12 +589,14 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             pf.launches[r.cls] += 1;
             pf.ms[r.cls] += ms;
         }
-        for (int c = 0; c < 3; ++c) {
-            pf.edges[c] = ctx->h_ctl->work_edges[c];
-            pf.found[c] = ctx->h_ctl->work_found[c];
+        if (P->engine != 0) {
+            for (int c = 0; c < 3; ++c) {
+                pf.edges[c] = ctx->h_ctl->work_edges[c];
+                pf.found[c] = ctx->h_ctl->work_found[c];
+            }
+            pf.edges[3] = 0;
+            pf.found[3] = fq0;
         }
-        pf.edges[3] = 0;
-        pf.found[3] = fq0;
     }
     ctx->have_solution = true;
     if (st) *st = local;
@@ -502,6 +607,14 @@ int bind_common(ispan_ctx* ctx, int32_t V, int64_t E) {
     ctx->V = V;
     ctx->E = E;
     ctx->nwords = ((int64_t)V + 63) / 64;
+    const size_t nv = (size_t)std::max(V, 1);
+    CK(hipMalloc(&ctx->fw_first, nv * 4));
+    CK(hipMalloc(&ctx->bw_first, nv * 4));
+    if (V > 0) {
+        CK(launch_first(ctx->fwb, ctx->fwa, V, ctx->fw_first, ctx->grid, ctx->stream));
+        CK(launch_first(ctx->bwb, ctx->bwa, V, ctx->bw_first, ctx->grid, ctx->stream));
+    }
+    CK(hipStreamSynchronize(ctx->stream));
     return ISPAN_OK;
 }
 
@@ -536,6 +649,7 @@ void ispan_default_params(ispan_params* p) {
     p->thread_count = 1;
     p->validate = 1;
     p->async_batch = 8;
+    p->engine = 1;
 }
 
 const char* ispan_strerror(int code) {
@@ -572,6 +686,17 @@ int ispan_ctx_create(ispan_ctx** out, int device) {
     if (hipDeviceGetAttribute(&ctx->cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return fail(ISPAN_ERR_HIP);
     ctx->grid = std::max(ctx->cu, 1) * 8;
+    {
+        int coop = 0, rate = 0;
+        (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+        (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device);
+        ctx->clock_khz = rate;
+        const int per_cu = coop ? std::min(giant_max_blocks_per_cu(), 8) : 0;
+        ctx->coop_grid = per_cu * ctx->cu;
+    }
+    if (hipMalloc(&ctx->gstate, sizeof(GiantState)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
+    if (hipHostMalloc(&ctx->h_gstate, sizeof(GiantState), hipHostMallocDefault) != hipSuccess)
+        return fail(ISPAN_ERR_NOMEM);
     if (hipMalloc(&ctx->ctl, sizeof(Ctl)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
     if (hipHostMalloc(&ctx->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
     for (auto& e : ctx->ev)
@@ -587,7 +712,9 @@ int ispan_ctx_destroy(ispan_ctx* ctx) {
     free_solve_buffers(ctx);
     free_graph(ctx);
     dfree(ctx->ctl);
+    dfree(ctx->gstate);
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
+    if (ctx->h_gstate) (void)hipHostFree(ctx->h_gstate);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->pool) (void)hipEventDestroy(e);
@@ -630,10 +757,12 @@ int ispan_graph_upload(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t* fw_b
     ctx->fwa = ctx->owned[1];
     ctx->bwb = ctx->owned[2];
     ctx->bwa = ctx->owned[3];
-    bind_common(ctx, V, E);
     ispan_params d;
     ispan_default_params(&d);
+    ctx->V = V;
+    ctx->E = E;
     int rc = check_bound(ctx, params ? params : &d);
+    if (!rc) rc = bind_common(ctx, V, E);
     if (rc) free_graph(ctx);
     return rc;
 }
@@ -648,10 +777,12 @@ int ispan_graph_bind_device(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t*
     ctx->fwa = d_fw_adj;
     ctx->bwb = d_bw_beg;
     ctx->bwa = d_bw_adj;
-    bind_common(ctx, V, E);
+    ctx->V = V;
+    ctx->E = E;
     ispan_params d;
     ispan_default_params(&d);
     int rc = check_bound(ctx, params ? params : &d);
+    if (!rc) rc = bind_common(ctx, V, E);
     if (rc) free_graph(ctx);
     return rc;
 }
