@@ -205,6 +205,17 @@ int ispan_build_csr_device(ispan_ctx* ctx, int32_t V, int64_t m, const int32_t* 
                            int64_t* E);
 void ispan_device_free(void* d_ptr);
 
+/* Text edge list ("src dst" pairs, leading '%' lines skipped, ids shifted by
+ * the smallest id) -> the reference's binary files in out_dir: fw_begin.bin,
+ * fw_adjacent.bin, head.bin, out_degree.bin (graph_converter/
+ * txt_to_bin_fw_int.cpp:31-217) and bw_begin.bin, bw_adjacent.bin,
+ * bw_head.bin, in_degree.bin (txt_to_bin_bw_int.cpp).  clean = 0 keeps the
+ * converters' layout (rows in input order, duplicates and self-loops kept);
+ * clean = 1 drops self-loops and duplicates and sorts every row.  The CSR is
+ * built on the device by stable radix sorts. */
+int ispan_convert_edge_list(ispan_ctx* ctx, const char* txt_path, const char* out_dir, int32_t clean,
+                            int32_t* V_out, int64_t* E_out);
+
 /* Library version string. */
 const char* ispan_version(void);
 

@@ -36,6 +36,7 @@ EXPORTS = [
     "ispan_labels_device", "ispan_solve_stats", "ispan_scc_detection", "ispan_graph_load", "ispan_graph_free",
     "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
     "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
+    "ispan_convert_edge_list",
 ]
 
 
@@ -142,6 +143,8 @@ def lib():
                                              C.POINTER(C.c_int64)]),
         "ispan_device_free": (None, [_vp]),
         "ispan_kernel_profile_get": (C.c_int, [_vp, C.POINTER(KernelProfile)]),
+        "ispan_convert_edge_list": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -339,6 +342,13 @@ class Solver:
         _check(lib().ispan_gen_grid_device(self._ctx, n, p_bidir, seed, *[C.byref(p) for p in ptrs], C.byref(E)),
                self._ctx)
         return DeviceGraph(n * n, E.value, *[C.cast(p, C.c_void_p).value or 0 for p in ptrs])
+
+    def convert_edge_list(self, txt_path: str, out_dir: str, clean: bool = False) -> tuple[int, int]:
+        """Text edge list -> the reference's binary files (graph_converter/)."""
+        V, E = C.c_int32(), C.c_int64()
+        _check(lib().ispan_convert_edge_list(self._ctx, txt_path.encode(), out_dir.encode(), int(bool(clean)),
+                                             C.byref(V), C.byref(E)), self._ctx)
+        return V.value, E.value
 
     def build_csr(self, V: int, m: int, d_src: int, d_dst: int) -> DeviceGraph:
         ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]

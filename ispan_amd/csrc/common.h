@@ -70,8 +70,10 @@ struct GiantState {
     GLine err;          // 1 barrier timeout, 2 level cap, 4 queue count out of range
     int64_t root, fq0, fw_count, giant, gmin, fq_n, fw_levels, bw_levels;
     int64_t work[4];    // adjacency entries read top-down / bottom-up, vertices found top-down / bottom-up
-    int64_t ts[5];      // wall-clock stamps: start, pivot known, FW done, BW done, end
-    int64_t pad[3];
+    int64_t ts[6];      // wall-clock stamps: start, pivot known, FW done, BW done, giant relabelled, end
+    GLine fqn[3];       // active-queue lengths of the in-kernel trim sweeps (rotating)
+    int64_t n_final, final_sel, trim1, pat2, pat3, sweeps, wcc_done;
+    int64_t pad[5];
 };
 
 struct GiantArgs {
@@ -97,12 +99,21 @@ struct GiantArgs {
     int64_t* part;            // kGiantSlots x grid partial counts
     int32_t* fq_n_out;        // receives the active queue length
     int64_t hub_cap;          // capacity of h0 / h1
+    // after the giant (when rest != 0): trim-1 sweeps, size-2/3 patterns,
+    // one more sweep and the WCC union-find run in the same launch
+    int32_t rest;
+    int32_t trim_times;
+    double theta;
+    int32_t* fq_alt;          // second active-queue buffer
+    int32_t* stamp;           // pattern size per labelled vertex
+    int32_t* parent;          // WCC union-find forest
+    int32_t* fq_n_ctl;        // Ctl::fq_n: receives the final queue length at [final buffer]
     int32_t alpha, beta, gamma;
     double avg_degree;
     int32_t wrap32;
 };
 
-constexpr int kGiantSlots = 12;
+constexpr int kGiantSlots = 16;
 constexpr int kGiantHubDeg = 512;   // degree from which a claimed vertex is split into work items
 constexpr int kGiantChunk = 2048;   // edges per work item
 inline int64_t giant_hub_capacity(int64_t E) { return 5 * (E / kGiantChunk) + 64; }
@@ -175,5 +186,7 @@ int gen_grid_device(hipStream_t s, int32_t n, double p_bidir, uint64_t seed, int
                     int32_t** bwb, int32_t** bwa, int64_t* E, std::string& err);
 int build_csr_device(hipStream_t s, int32_t V, int64_t m, const int32_t* src, const int32_t* dst, int32_t** fwb,
                      int32_t** fwa, int32_t** bwb, int32_t** bwa, int64_t* E, std::string& err);
+int csr_stable_one(hipStream_t s, int32_t V, int64_t m, const int32_t* key, const int32_t* val, int32_t** beg,
+                   int32_t** adj, std::string& err);
 
 }  // namespace ispan

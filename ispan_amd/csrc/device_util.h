@@ -155,5 +155,82 @@ __device__ __forceinline__ bool pivot_better(int64_t p, int32_t i, int64_t bp, i
 }
 
 
+// ---------------------------------------------------------- size-2/3 trims
+// trim_2_from_fq (src/trim_2_3.h:146-283), trim_3_1_from_fq (:482-677) and
+// trim_3_2_from_fq (:881-1080), checked against act as it stands when the
+// pass starts: detections only write lab (and the pattern size into
+// stamp); act is cleared afterwards.  With a fixed active set every
+// detected group is an SCC of the active subgraph, so two detections of one
+// vertex always agree on its label.
+
+// Distinct active neighbours of x other than x, e1, e2: returns how many
+// were seen, stopping at cap + 1; the first (up to 2) are in g0, g1.
+__device__ __forceinline__ int nb_scan(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
+                                       const uint64_t* __restrict__ act, int32_t x, int32_t e1, int32_t e2, int cap,
+                                       int32_t& g0, int32_t& g1) {
+    int n = 0;
+    const int32_t end = beg[x + 1];
+    for (int32_t k = beg[x]; k < end; ++k) {
+        const int32_t w = adj[k];
+        if (w == x || w == e1 || w == e2 || !bit_of(act, w)) continue;
+        if ((n >= 1 && w == g0) || (n >= 2 && w == g1)) continue;
+        if (n == 0) g0 = w;
+        else if (n == 1) g1 = w;
+        if (++n > cap) break;
+    }
+    return n;
+}
+
+__device__ __forceinline__ void pattern_mark(int32_t* lab, int32_t* stamp, int32_t a, int32_t b, int32_t c, int size) {
+    int32_t m = min(a, b);
+    if (size == 3) m = min(m, c);
+    lab[a] = m;
+    lab[b] = m;
+    stamp[a] = size;
+    stamp[b] = size;
+    if (size == 3) {
+        lab[c] = m;
+        stamp[c] = size;
+    }
+}
+
+// Checks vertex a against the size-2 pattern, then the size-3 cycle, then
+// the size-3 hub, on out-edges and then in-edges; marks the first match.
+__device__ __forceinline__ void pattern_check(const int32_t* __restrict__ fwb, const int32_t* __restrict__ fwa,
+                                              const int32_t* __restrict__ bwb, const int32_t* __restrict__ bwa,
+                                              int32_t* lab, int32_t* stamp, const uint64_t* __restrict__ act,
+                                              int32_t a) {
+    for (int dir = 0; dir < 2; ++dir) {
+        const int32_t* beg = dir == 0 ? fwb : bwb;
+        const int32_t* adj = dir == 0 ? fwa : bwa;
+        int32_t g0 = -1, g1 = -1, h0 = -1, h1 = -1, k0 = -1, k1 = -1;
+        const int na = nb_scan(beg, adj, act, a, -1, -1, 2, g0, g1);
+        if (na == 1) {
+            const int32_t b = g0;
+            // size 2: b's only other one is a
+            if (nb_scan(beg, adj, act, b, -1, -1, 1, h0, h1) == 1 && h0 == a) {
+                pattern_mark(lab, stamp, a, b, -1, 2);
+                return;
+            }
+            // size 3 cycle: a -> b -> c -> a
+            if (nb_scan(beg, adj, act, b, a, -1, 1, h0, h1) == 1) {
+                const int32_t c = h0;
+                if (nb_scan(beg, adj, act, c, b, -1, 1, k0, k1) == 1 && k0 == a) {
+                    pattern_mark(lab, stamp, a, b, c, 3);
+                    return;
+                }
+            }
+        } else if (na == 2) {
+            // size 3 hub: a <-> b, a <-> c
+            const int32_t b = g0, c = g1;
+            if (nb_scan(beg, adj, act, b, b, c, 1, h0, h1) == 1 && h0 == a &&
+                nb_scan(beg, adj, act, c, b, c, 1, k0, k1) == 1 && k0 == a) {
+                pattern_mark(lab, stamp, a, b, c, 3);
+                return;
+            }
+        }
+    }
+}
+
 }  // namespace dev
 }  // namespace ispan

@@ -127,6 +127,25 @@ __global__ void k_csr_from_keys(const uint64_t* __restrict__ keys, int64_t E, in
     }
 }
 
+__global__ void k_widen_keys(const uint32_t* __restrict__ k, int64_t m, uint64_t* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (uint64_t)k[i] << 32;
+}
+
+// beg[v] = first i with keys[i] >= v<<32 over sorted keys.
+__global__ void k_beg_from_keys(const uint64_t* __restrict__ keys, int64_t m, int32_t V, int32_t* beg) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = (uint64_t)v << 32;
+        int64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        beg[v] = (int32_t)lo;
+    }
+}
+
 __global__ void k_swap_halves(const uint64_t* __restrict__ in, int64_t E, uint64_t* out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t k = in[i];
@@ -212,6 +231,50 @@ done:
             if (*p) (void)hipFree(*p);
             *p = nullptr;
         }
+    }
+    return rc;
+}
+
+// Stable CSR of one direction: rows in input order, duplicates and
+// self-loops kept (the byte layout of graph_converter/txt_to_bin_fw_int.cpp
+// :146-181).  key[i] = row of edge i, val[i] = its other endpoint.
+int csr_stable_one(hipStream_t s, int32_t V, int64_t m, const int32_t* key, const int32_t* val, int32_t** beg,
+                   int32_t** adj, std::string& err) {
+    int rc = 0;
+    uint32_t *k2 = nullptr, *kin = nullptr;
+    int32_t* v2 = nullptr;
+    void* tmp = nullptr;
+    size_t bytes = 0;
+    uint64_t* keys64 = nullptr;
+    *beg = *adj = nullptr;
+    unsigned end_bit = 1;
+    while (end_bit < 32 && ((int64_t)1 << end_bit) < (int64_t)V) ++end_bit;
+    GK(hipMalloc(&kin, (size_t)std::max<int64_t>(m, 1) * 4));
+    GK(hipMalloc(&k2, (size_t)std::max<int64_t>(m, 1) * 4));
+    GK(hipMalloc(&v2, (size_t)std::max<int64_t>(m, 1) * 4));
+    GK(hipMalloc(adj, (size_t)std::max<int64_t>(m, 1) * 4));
+    GK(hipMalloc(beg, ((size_t)V + 1) * 4));
+    GK(hipMalloc(&keys64, (size_t)std::max<int64_t>(m, 1) * 8));
+    if (m > 0) {
+        GK(hipMemcpyAsync(kin, key, (size_t)m * 4, hipMemcpyDeviceToDevice, s));
+        GK(rocprim::radix_sort_pairs(nullptr, bytes, kin, k2, val, v2, (size_t)m, 0, end_bit, s));
+        GK(hipMalloc(&tmp, bytes + 256));
+        bytes += 256;
+        GK(rocprim::radix_sort_pairs(tmp, bytes, kin, k2, val, v2, (size_t)m, 0, end_bit, s));
+        GK(hipMemcpyAsync(*adj, v2, (size_t)m * 4, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_widen_keys, dim3(kGrid), dim3(256), 0, s, k2, (int64_t)m, keys64);
+        GK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_beg_from_keys, dim3(kGrid), dim3(256), 0, s, keys64, (int64_t)m, V, *beg);
+    GK(hipGetLastError());
+    GK(hipStreamSynchronize(s));
+done:
+    for (void* p : {(void*)kin, (void*)k2, (void*)v2, tmp, (void*)keys64})
+        if (p) (void)hipFree(p);
+    if (rc) {
+        if (*beg) (void)hipFree(*beg);
+        if (*adj) (void)hipFree(*adj);
+        *beg = *adj = nullptr;
     }
     return rc;
 }

@@ -21,7 +21,10 @@ constexpr int kBarGroups = 32;     // arrival counters of the two-level barrier
 constexpr int kW = kBlock / kWave;
 
 // Partial-count slots: part[slot * grid + block].
-enum Slot { kPivP, kPivI, kSurv, kFound0, kFound1, kMin0, kMin1, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
+enum Slot {
+    kPivP, kPivI, kSurv, kFound0, kFound1, kMin0, kMin1, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU, kTrim0, kTrim1,
+    kPat2, kPat3
+};
 static_assert(kFoundBU < kGiantSlots, "slot count");
 
 struct Smem {
@@ -331,6 +334,156 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
     return visited;
 }
 
+// ---- phases after the giant (run when GiantArgs::rest != 0) --------------
+
+// One trim-1 sweep (trim_1_from_fq_gfq, src/trim_1_gfq.h:365-458) over
+// fin[0, n): act is updated in place, survivors are pushed to fout.
+// Returns the number trimmed (block partial written to `slot`).
+__device__ void trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount, int slot) {
+    const GiantArgs& a = c.a;
+    WaveQueue wq{c.sm.wbuf[c.wave], 0};
+    const uint64_t* act = a.act;
+    auto live = [act](int32_t w, int32_t self) { return w != self && bit_of(act, w); };
+    int trimmed = 0, scanned = 0;
+    for (int64_t ib = (int64_t)c.b * kBlock + c.wave * kWave; ib < n; ib += (int64_t)c.G * kBlock) {
+        const int64_t i = ib + c.lane;
+        int32_t v = 0;
+        bool alive = false;
+        if (i < n) {
+            v = load_agent(&fin[i]);
+            alive = bit_of(a.act, v);
+        }
+        const bool out = wave_any_edge(alive, alive ? a.fwb[v] : 0, alive ? a.fwb[v + 1] : 0, v, a.fwa, live, scanned);
+        const bool in =
+            wave_any_edge(alive && out, alive ? a.bwb[v] : 0, alive ? a.bwb[v + 1] : 0, v, a.bwa, live, scanned);
+        const bool trim = alive && !(out && in);
+        if (trim) {
+            a.lab[v] = v;
+            atomicAnd(reinterpret_cast<unsigned long long*>(a.act + (v >> 6)), ~(1ull << (v & 63)));
+            ++trimmed;
+        }
+        wq_push(wq, alive && !trim, v, fout, fcount);
+    }
+    wq_flush(wq, fout, fcount);
+    const int64_t t = block_sum(trimmed, c.sm);
+    if (c.tid == 0) a.part[(int64_t)slot * c.G + c.b] = t;
+}
+
+__device__ int32_t uf_root_g(int32_t* parent, int32_t v) {
+    int32_t nx = load_agent(&parent[v]);
+    while (nx != v) {
+        v = nx;
+        nx = load_agent(&parent[v]);
+    }
+    return v;
+}
+
+// Trims and WCC after the giant.  Returns false after a failed barrier.
+__device__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
+    const GiantArgs& a = c.a;
+    GiantState* gs = a.gs;
+    int32_t* buf[2] = {a.fq_alt, a.fq};
+    int cur = 1, s = 0;
+    int64_t prev = fq0, trim1 = 0, p2 = 0, p3 = 0;
+    int it = 0;
+    const int64_t nthreads = (int64_t)c.G * kBlock;
+    const int64_t gt = (int64_t)c.b * kBlock + c.tid;
+    auto sweep = [&]() -> bool {
+        if (c.b == 0 && c.tid == 0)
+            __hip_atomic_store(&gs->fqn[(s + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int slot = (s & 1) ? kTrim1 : kTrim0;
+        trim_sweep(c, buf[cur], n, buf[cur ^ 1], &gs->fqn[s % 3].v, slot);
+        if (!gsync(gs, c.G, c.sm)) return false;
+        trim1 += sum_slot(a.part, slot, c.G, c.sm);
+        n = load_agent(&gs->fqn[s % 3].v);
+        cur ^= 1;
+        ++s;
+        return true;
+    };
+    // P7: up to trim_times sweeps with the theta stop rule (src/scc_core.cpp:562-602);
+    // the first comparison is against the pre-giant queue, as there.
+    while (prev > 0 && it < a.trim_times && n > 0) {
+        if (!sweep()) return false;
+        if (prev - n < prev * a.theta) break;
+        prev = n;
+        ++it;
+    }
+    bool wcc = false;
+    if (n > 0) {
+        // P8-P10: pattern detection against the current act, then commit
+        const int32_t* f = buf[cur];
+        for (int64_t i = gt; i < n; i += nthreads) {
+            const int32_t v = load_agent(&f[i]);
+            if (bit_of(a.act, v)) pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v);
+        }
+        if (!gsync(gs, c.G, c.sm)) return false;
+        int c2 = 0, c3 = 0;
+        for (int64_t i = gt; i < n; i += nthreads) {
+            const int32_t v = load_agent(&f[i]);
+            if (load_agent(&a.lab[v]) >= 0 && bit_of(a.act, v)) {
+                atomicAnd(reinterpret_cast<unsigned long long*>(a.act + (v >> 6)), ~(1ull << (v & 63)));
+                if (load_agent(&a.stamp[v]) == 2) ++c2;
+                else ++c3;
+            }
+        }
+        const int64_t b2 = block_sum(c2, c.sm), b3 = block_sum(c3, c.sm);
+        if (c.tid == 0) {
+            a.part[(int64_t)kPat2 * c.G + c.b] = b2;
+            a.part[(int64_t)kPat3 * c.G + c.b] = b3;
+        }
+        if (!gsync(gs, c.G, c.sm)) return false;
+        p2 = sum_slot(a.part, kPat2, c.G, c.sm);
+        p3 = sum_slot(a.part, kPat3, c.G, c.sm);
+        // P11: one more trim-1 sweep
+        if (!sweep()) return false;
+        if (n > 0) {
+            // P12: WCC by union-find; parent = smallest id of the WCC
+            const int32_t* g = buf[cur];
+            for (int64_t i = gt; i < n; i += nthreads) {
+                const int32_t v = load_agent(&g[i]);
+                a.parent[v] = v;
+            }
+            if (!gsync(gs, c.G, c.sm)) return false;
+            for (int64_t i = gt; i < n; i += nthreads) {
+                const int32_t v = load_agent(&g[i]);
+                for (int32_t k = a.fwb[v]; k < a.fwb[v + 1]; ++k) {
+                    const int32_t w = a.fwa[k];
+                    if (w == v || !bit_of(a.act, w)) continue;
+                    int32_t x = v, y = w;
+                    while (true) {
+                        x = uf_root_g(a.parent, x);
+                        y = uf_root_g(a.parent, y);
+                        if (x == y) break;
+                        if (x < y) {
+                            const int32_t t = x;
+                            x = y;
+                            y = t;
+                        }
+                        if (atomicCAS(&a.parent[x], x, y) == x) break;
+                    }
+                }
+            }
+            if (!gsync(gs, c.G, c.sm)) return false;
+            for (int64_t i = gt; i < n; i += nthreads) {
+                const int32_t v = load_agent(&g[i]);
+                a.parent[v] = uf_root_g(a.parent, v);
+            }
+            wcc = true;
+        }
+    }
+    if (c.b == 0 && c.tid == 0) {
+        gs->n_final = n;
+        gs->final_sel = cur;
+        gs->trim1 = trim1;
+        gs->pat2 = p2;
+        gs->pat3 = p3;
+        gs->sweeps = s;
+        gs->wcc_done = wcc;
+        a.fq_n_ctl[cur] = (int32_t)n;
+    }
+    return true;
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
@@ -515,6 +668,11 @@ __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
             *a.fq_n_out = (int32_t)tot;
             gs->ts[4] = wall_clock64();
         }
+    }
+    if (a.rest) {
+        if (!gsync(gs, c.G, sm)) return;  // queue written by every block
+        if (!rest_phase(c, fq0, tot)) return;
+        if (stamp) gs->ts[5] = wall_clock64();
     }
 }
 

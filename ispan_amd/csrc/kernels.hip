@@ -458,37 +458,6 @@ __global__ __launch_bounds__(kBlock) void k_trim1(const int32_t* __restrict__ fw
 // active set every detected group is an SCC of the active subgraph, so two
 // detections of one vertex always agree on its label.
 
-// Distinct active neighbours of x other than x, e1, e2: returns how many
-// were seen, stopping at cap + 1; the first (up to 2) are in g0, g1.
-__device__ __forceinline__ int nb_scan(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
-                                       const uint64_t* __restrict__ act, int32_t x, int32_t e1, int32_t e2, int cap,
-                                       int32_t& g0, int32_t& g1) {
-    int n = 0;
-    const int32_t end = beg[x + 1];
-    for (int32_t k = beg[x]; k < end; ++k) {
-        const int32_t w = adj[k];
-        if (w == x || w == e1 || w == e2 || !bit_of(act, w)) continue;
-        if ((n >= 1 && w == g0) || (n >= 2 && w == g1)) continue;
-        if (n == 0) g0 = w;
-        else if (n == 1) g1 = w;
-        if (++n > cap) break;
-    }
-    return n;
-}
-
-__device__ __forceinline__ void pattern_mark(int32_t* lab, int32_t* stamp, int32_t a, int32_t b, int32_t c, int size) {
-    int32_t m = min(a, b);
-    if (size == 3) m = min(m, c);
-    lab[a] = m;
-    lab[b] = m;
-    stamp[a] = size;
-    stamp[b] = size;
-    if (size == 3) {
-        lab[c] = m;
-        stamp[c] = size;
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_pattern(const int32_t* __restrict__ fwb, const int32_t* __restrict__ fwa,
                                                     const int32_t* __restrict__ bwb, const int32_t* __restrict__ bwa,
                                                     int32_t* lab, int32_t* stamp, const uint64_t* __restrict__ act,
@@ -496,37 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_pattern(const int32_t* __restrict__ 
     const int n = ctl->fq_n[sel];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = fq[i];
-        if (!bit_of(act, a)) continue;
-        for (int dir = 0; dir < 2; ++dir) {
-            const int32_t* beg = dir == 0 ? fwb : bwb;
-            const int32_t* adj = dir == 0 ? fwa : bwa;
-            int32_t g0 = -1, g1 = -1, h0 = -1, h1 = -1, k0 = -1, k1 = -1;
-            const int na = nb_scan(beg, adj, act, a, -1, -1, 2, g0, g1);
-            if (na == 1) {
-                const int32_t b = g0;
-                // size 2: b's only other one is a
-                if (nb_scan(beg, adj, act, b, -1, -1, 1, h0, h1) == 1 && h0 == a) {
-                    pattern_mark(lab, stamp, a, b, -1, 2);
-                    break;
-                }
-                // size 3 cycle: a -> b -> c -> a
-                if (nb_scan(beg, adj, act, b, a, -1, 1, h0, h1) == 1) {
-                    const int32_t c = h0;
-                    if (nb_scan(beg, adj, act, c, b, -1, 1, k0, k1) == 1 && k0 == a) {
-                        pattern_mark(lab, stamp, a, b, c, 3);
-                        break;
-                    }
-                }
-            } else if (na == 2) {
-                // size 3 hub: a <-> b, a <-> c
-                const int32_t b = g0, c = g1;
-                if (nb_scan(beg, adj, act, b, b, c, 1, h0, h1) == 1 && h0 == a &&
-                    nb_scan(beg, adj, act, c, b, c, 1, k0, k1) == 1 && k0 == a) {
-                    pattern_mark(lab, stamp, a, b, c, 3);
-                    break;
-                }
-            }
-        }
+        if (bit_of(act, a)) pattern_check(fwb, fwa, bwb, bwa, lab, stamp, act, a);
     }
 }
 

@@ -345,7 +345,8 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     }
     int64_t fq0 = 0, n = 0, fw_count = 0;
     int32_t root = -1;
-    double giant_phase_ms[3] = {0, 0, 0};  // engine 0: trim-1-first+pivot, FW, BW+relabel
+    bool rest_done = false;
+    double giant_phase_ms[4] = {0, 0, 0, 0};  // engine 0: trim-1-first+pivot, FW, BW+relabel, trims+WCC
     if (P->engine == 0) {
         // P1-P6 + relabel in one cooperative kernel
         if (ctx->coop_grid <= 0) {
@@ -380,6 +381,13 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.gamma = P->gamma;
         ga.avg_degree = avg_degree;
         ga.wrap32 = P->pivot_wrap32;
+        ga.rest = 1;
+        ga.trim_times = P->trim_times;
+        ga.theta = P->theta;
+        ga.fq_alt = ctx->fq[0];
+        ga.stamp = ctx->stamp;
+        ga.parent = ctx->parent;
+        ga.fq_n_ctl = ctx->ctl->fq_n;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
         if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) {
             ctx->recs.push_back({3, ga0, ga1});
@@ -415,11 +423,14 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             giant_phase_ms[0] = (gs.ts[1] - gs.ts[0]) * k;
             giant_phase_ms[1] = root >= 0 ? (gs.ts[2] - gs.ts[1]) * k : 0.0;
             giant_phase_ms[2] = (gs.ts[4] - (root >= 0 ? gs.ts[2] : gs.ts[1])) * k;
+            giant_phase_ms[3] = gs.ts[5] > gs.ts[4] ? (gs.ts[5] - gs.ts[4]) * k : 0.0;
         }
+        rest_done = true;
         ctx->prof.edges[0] = gs.work[0];
         ctx->prof.edges[2] = gs.work[1];
         ctx->prof.found[0] = gs.work[2];
         ctx->prof.found[2] = gs.work[3];
+        ctx->prof.found[3] = fq0;
     } else {
         // P1-P3: trim-1-first, active queue, pivot
         hipEvent_t t1a = nullptr, t1b = nullptr;
@@ -460,11 +471,22 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         local.giant_size = root >= 0 ? fq0 - n : 0;  // giant = removed by post_giant
     }
     int sel = 1;
-    int64_t trim1 = 0;
-
-    // P7 trim-1 sweeps with the theta stop rule (src/scc_core.cpp:562-602):
-    // the first comparison is against the pre-giant queue, as there.
-    {
+    int64_t trim1 = 0, pat2 = 0, pat3 = 0;
+    bool wcc_ready = false;
+    if (rest_done) {
+        const GiantState& gs = *ctx->h_gstate;
+        sel = (int)gs.final_sel;
+        n = gs.n_final;
+        trim1 = gs.trim1;
+        pat2 = gs.pat2;
+        pat3 = gs.pat3;
+        wcc_ready = gs.wcc_done != 0;
+        rec(ctx, P, 5);
+        rec(ctx, P, 6);
+        rec(ctx, P, 7);
+    } else {
+        // P7 trim-1 sweeps with the theta stop rule (src/scc_core.cpp:562-602):
+        // the first comparison is against the pre-giant queue, as there.
         int64_t prev = fq0;
         int it = 0;
         while (prev > 0 && it < P->trim_times) {
@@ -480,61 +502,59 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             prev = nn;
             ++it;
         }
-    }
-    rec(ctx, P, 5);
-    int64_t pat2 = 0, pat3 = 0;
-    if (n > 0) {
-        // P8-P10 size-2/3 pattern trims
-        CK(launch_pattern_trim(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->act, ctx->fq[sel],
-                               ctx->ctl, sel, G, s));
-        CK(launch_pattern_commit(ctx->lab, ctx->act, ctx->stamp, ctx->fq[sel], ctx->ctl, sel, G, s));
-        rec(ctx, P, 6);
-        // P11 one trim-1 sweep
-        CK(launch_trim1_sweep(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->act, ctx->fq, ctx->ctl, sel, G,
-                              s));
-        sel ^= 1;
-        rec(ctx, P, 7);
-        if ((rc = readback(ctx, &syncs))) return rc;
-        n = ctx->h_ctl->fq_n[sel];
-        trim1 += ctx->h_ctl->trimmed;
-        pat2 = ctx->h_ctl->pattern2;
-        pat3 = ctx->h_ctl->pattern3;
+        rec(ctx, P, 5);
         if (n > 0) {
-            // P12 WCC of the remaining subgraph
-            if (P->wcc_mode == 1) {
-                CK(launch_uf_init(ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
-                int it = 0;
-                for (;; ++it) {
-                    if (it > 4 * 1024 * 1024) {
-                        ctx->err = "label propagation did not converge";
-                        return ISPAN_ERR_ITER;
-                    }
-                    CK(launch_lp_round(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->act, ctx->fq[sel], ctx->parent,
-                                       ctx->ctl, sel, G, s));
-                    if ((rc = readback(ctx, &syncs))) return rc;
-                    if (!ctx->h_ctl->changed) break;
-                }
-            } else {
-                CK(launch_wcc_uf(ctx->fwb, ctx->fwa, ctx->act, ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
-            }
-            rec(ctx, P, 8);
-            // P13-P14 mice: group by WCC, FW-BW per WCC
-            CK(launch_mice_keys(ctx->fq[sel], ctx->parent, ctx->stamp, ctx->keys, ctx->ctl, sel, G, s));
-            int end_bit = 32;
-            while (end_bit < 64 && ((int64_t)1 << (end_bit - 32)) < (int64_t)V) ++end_bit;
-            size_t bytes = ctx->sort_tmp_bytes;
-            CK(rocprim::radix_sort_keys(ctx->sort_tmp, bytes, ctx->keys, ctx->keys_alt, (size_t)n, 0,
-                                        (unsigned)end_bit, s));
-            CK(launch_mice_starts(ctx->keys_alt, (int32_t)n, ctx->starts, ctx->ctl, G, s));
-            CK(launch_mice(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->keys_alt, (int32_t)n,
-                           ctx->starts, ctx->ctl, ctx->qbuf, G, s));
-            local.mice = n;
+            // P8-P10 size-2/3 pattern trims
+            CK(launch_pattern_trim(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->act,
+                                   ctx->fq[sel], ctx->ctl, sel, G, s));
+            CK(launch_pattern_commit(ctx->lab, ctx->act, ctx->stamp, ctx->fq[sel], ctx->ctl, sel, G, s));
+            rec(ctx, P, 6);
+            // P11 one trim-1 sweep
+            CK(launch_trim1_sweep(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->act, ctx->fq, ctx->ctl, sel,
+                                  G, s));
+            sel ^= 1;
+            rec(ctx, P, 7);
+            if ((rc = readback(ctx, &syncs))) return rc;
+            n = ctx->h_ctl->fq_n[sel];
+            trim1 += ctx->h_ctl->trimmed;
+            pat2 = ctx->h_ctl->pattern2;
+            pat3 = ctx->h_ctl->pattern3;
         } else {
-            rec(ctx, P, 8);
+            rec(ctx, P, 6);
+            rec(ctx, P, 7);
         }
+    }
+    if (n > 0) {
+        // P12 WCC of the remaining subgraph
+        if (P->wcc_mode == 1) {
+            CK(launch_uf_init(ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
+            int it = 0;
+            for (;; ++it) {
+                if (it > 4 * 1024 * 1024) {
+                    ctx->err = "label propagation did not converge";
+                    return ISPAN_ERR_ITER;
+                }
+                CK(launch_lp_round(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->act, ctx->fq[sel], ctx->parent,
+                                   ctx->ctl, sel, G, s));
+                if ((rc = readback(ctx, &syncs))) return rc;
+                if (!ctx->h_ctl->changed) break;
+            }
+        } else if (!wcc_ready) {
+            CK(launch_wcc_uf(ctx->fwb, ctx->fwa, ctx->act, ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
+        }
+        rec(ctx, P, 8);
+        // P13-P14 mice: group by WCC, FW-BW per WCC
+        CK(launch_mice_keys(ctx->fq[sel], ctx->parent, ctx->stamp, ctx->keys, ctx->ctl, sel, G, s));
+        int end_bit = 32;
+        while (end_bit < 64 && ((int64_t)1 << (end_bit - 32)) < (int64_t)V) ++end_bit;
+        size_t bytes = ctx->sort_tmp_bytes;
+        CK(rocprim::radix_sort_keys(ctx->sort_tmp, bytes, ctx->keys, ctx->keys_alt, (size_t)n, 0, (unsigned)end_bit,
+                                    s));
+        CK(launch_mice_starts(ctx->keys_alt, (int32_t)n, ctx->starts, ctx->ctl, G, s));
+        CK(launch_mice(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->keys_alt, (int32_t)n,
+                       ctx->starts, ctx->ctl, ctx->qbuf, G, s));
+        local.mice = n;
     } else {
-        rec(ctx, P, 6);
-        rec(ctx, P, 7);
         rec(ctx, P, 8);
     }
     rec(ctx, P, 9);
@@ -564,12 +584,13 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             fw = giant_phase_ms[1];
             bw = giant_phase_ms[2];
         }
+        const double rest_ms = P->engine == 0 ? giant_phase_ms[3] : 0.0;
         const double
                      t1 = ev_ms(ctx, 4, 5), t23 = ev_ms(ctx, 5, 6), t1b = ev_ms(ctx, 6, 7), wcc = ev_ms(ctx, 7, 8),
                      mice = ev_ms(ctx, 8, 9);
         local.ms[11] = first;
-        local.ms[14] = t1 + t1b;
-        local.ms[4] = first + t1 + t1b;
+        local.ms[14] = t1 + t1b + rest_ms;  // engine 0: in-kernel sweeps, patterns and WCC
+        local.ms[4] = first + t1 + t1b + rest_ms;
         local.ms[5] = t23;
         local.ms[13] = 0.0;  // size-3 patterns run fused with size-2 (ms[5])
         local.ms[6] = 0.0;   // pivot reduction is fused into trim-1-first (ms[11])
@@ -952,6 +973,152 @@ int ispan_build_csr_device(ispan_ctx* ctx, int32_t V, int64_t m, const int32_t* 
 int ispan_kernel_profile_get(const ispan_ctx* ctx, ispan_kernel_profile* out) {
     if (!ctx || !out) return ISPAN_ERR_ARG;
     *out = ctx->prof;
+    return ISPAN_OK;
+}
+
+// Text edge list -> the reference's binary files (graph_converter/
+// txt_to_bin_fw_int.cpp:31-217 and txt_to_bin_bw_int.cpp).  Leading lines
+// starting with '%' are skipped; ids are shifted by the smallest id.
+static int parse_edge_list(const char* path, std::vector<int32_t>& src, std::vector<int32_t>& dst, int32_t* V,
+                           std::string& err) {
+    FILE* f = fopen(path, "rb");
+    if (!f) {
+        err = std::string("cannot open ") + path;
+        return ISPAN_ERR_IO;
+    }
+    std::vector<char> buf;
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    fclose(f);
+    size_t i = 0;
+    const size_t n = buf.size();
+    while (i < n && buf[i] == '%') {
+        while (i < n && buf[i] != '\n') ++i;
+        ++i;
+    }
+    std::vector<int64_t> ids;
+    ids.reserve(n / 4);
+    while (i < n) {
+        while (i < n && (buf[i] == ' ' || buf[i] == '\t' || buf[i] == '\n' || buf[i] == '\r')) ++i;
+        if (i >= n) break;
+        bool neg = false;
+        if (buf[i] == '-') {
+            neg = true;
+            ++i;
+        }
+        if (i >= n || buf[i] < '0' || buf[i] > '9') {
+            err = "edge list: malformed token at byte " + std::to_string(i);
+            return ISPAN_ERR_IO;
+        }
+        int64_t x = 0;
+        while (i < n && buf[i] >= '0' && buf[i] <= '9') {
+            x = x * 10 + (buf[i] - '0');
+            if (x > INT32_MAX) {
+                err = "edge list: id exceeds int32";
+                return ISPAN_ERR_IO;
+            }
+            ++i;
+        }
+        ids.push_back(neg ? -x : x);
+    }
+    if (ids.size() % 2) {
+        err = "edge list: odd number of ids";
+        return ISPAN_ERR_IO;
+    }
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t x : ids) {
+        lo = std::min(lo, x);
+        hi = std::max(hi, x);
+    }
+    if (ids.empty()) lo = hi = 0;
+    if (hi - lo + 1 > INT32_MAX - 1) {
+        err = "edge list: id range exceeds int32";
+        return ISPAN_ERR_IO;
+    }
+    *V = ids.empty() ? 0 : (int32_t)(hi - lo + 1);
+    const size_t m = ids.size() / 2;
+    src.resize(m);
+    dst.resize(m);
+    for (size_t k = 0; k < m; ++k) {
+        src[k] = (int32_t)(ids[2 * k] - lo);
+        dst[k] = (int32_t)(ids[2 * k + 1] - lo);
+    }
+    return ISPAN_OK;
+}
+
+static int write_i32(const std::string& path, const int32_t* p, size_t count) {
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return ISPAN_ERR_IO;
+    const size_t w = count ? fwrite(p, 4, count, f) : 0;
+    fclose(f);
+    return w == count ? ISPAN_OK : ISPAN_ERR_IO;
+}
+
+int ispan_convert_edge_list(ispan_ctx* ctx, const char* txt_path, const char* out_dir, int32_t clean, int32_t* V_out,
+                            int64_t* E_out) {
+    if (!ctx || !txt_path || !out_dir) return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    std::vector<int32_t> src, dst;
+    int32_t V = 0;
+    int rc = parse_edge_list(txt_path, src, dst, &V, ctx->err);
+    if (rc) return rc;
+    const int64_t m = (int64_t)src.size();
+    int32_t *d_src = nullptr, *d_dst = nullptr, *fb = nullptr, *fa = nullptr, *bb = nullptr, *ba = nullptr;
+    int64_t E = m;
+    std::vector<int32_t> h[4];
+    CK(hipMalloc(&d_src, (size_t)std::max<int64_t>(m, 1) * 4));
+    CK(hipMalloc(&d_dst, (size_t)std::max<int64_t>(m, 1) * 4));
+    if (m) {
+        CK(hipMemcpy(d_src, src.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+        CK(hipMemcpy(d_dst, dst.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+    }
+    if (clean) {
+        rc = gen_rc(build_csr_device(ctx->stream, V, m, d_src, d_dst, &fb, &fa, &bb, &ba, &E, ctx->err));
+    } else {
+        rc = gen_rc(csr_stable_one(ctx->stream, V, m, d_src, d_dst, &fb, &fa, ctx->err));
+        if (!rc) rc = gen_rc(csr_stable_one(ctx->stream, V, m, d_dst, d_src, &bb, &ba, ctx->err));
+    }
+    (void)hipFree(d_src);
+    (void)hipFree(d_dst);
+    if (!rc) {
+        int32_t* dp[4] = {fb, fa, bb, ba};
+        const size_t cnt[4] = {(size_t)V + 1, (size_t)E, (size_t)V + 1, (size_t)E};
+        for (int k = 0; k < 4 && !rc; ++k) {
+            h[k].resize(std::max<size_t>(cnt[k], 1));
+            if (cnt[k] && hipMemcpy(h[k].data(), dp[k], cnt[k] * 4, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = ISPAN_ERR_HIP;
+        }
+    }
+    for (int32_t* p : {fb, fa, bb, ba})
+        if (p) (void)hipFree(p);
+    if (rc) return rc;
+    // side files of the reference converters: source/destination per edge and degrees
+    std::vector<int32_t> head((size_t)std::max<int64_t>(E, 1)), bhead((size_t)std::max<int64_t>(E, 1));
+    std::vector<int32_t> odeg((size_t)std::max(V, 1)), ideg((size_t)std::max(V, 1));
+    for (int32_t v = 0; v < V; ++v) {
+        odeg[v] = h[0][v + 1] - h[0][v];
+        ideg[v] = h[2][v + 1] - h[2][v];
+        for (int32_t k = h[0][v]; k < h[0][v + 1]; ++k) head[k] = v;
+        for (int32_t k = h[2][v]; k < h[2][v + 1]; ++k) bhead[k] = v;
+    }
+    const std::string d(out_dir);
+    const struct {
+        const char* name;
+        const int32_t* p;
+        size_t n;
+    } files[] = {{"fw_begin.bin", h[0].data(), (size_t)V + 1}, {"fw_adjacent.bin", h[1].data(), (size_t)E},
+                 {"head.bin", head.data(), (size_t)E},          {"out_degree.bin", odeg.data(), (size_t)V},
+                 {"bw_begin.bin", h[2].data(), (size_t)V + 1}, {"bw_adjacent.bin", h[3].data(), (size_t)E},
+                 {"bw_head.bin", bhead.data(), (size_t)E},      {"in_degree.bin", ideg.data(), (size_t)V}};
+    for (const auto& fl : files) {
+        if (write_i32(d + "/" + fl.name, fl.p, fl.n)) {
+            ctx->err = std::string("cannot write ") + d + "/" + fl.name;
+            return ISPAN_ERR_IO;
+        }
+    }
+    if (V_out) *V_out = V;
+    if (E_out) *E_out = E;
     return ISPAN_OK;
 }
 

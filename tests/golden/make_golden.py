@@ -116,6 +116,7 @@ def pattern_graphs():
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true")
+    ap.add_argument("--toy-only", action="store_true")
     a = ap.parse_args(argv)
     O.build()
 
@@ -133,10 +134,15 @@ def main(argv=None) -> int:
            "canonical": O.tarjan(g).tolist(), "ref_raw_scc_id": raw.tolist(),
            "ref_result_block": [l for l in out.splitlines() if "," in l and l.split(",")[0] in
                                 ("largest", "trimmed size_1", "trimmed size_2", "trimmed size_3", "others",
-                                 "total")]}
+                                 "total")],
+           "edge_list_text": open(os.path.join(REF_TOY, "toy_edge_list.txt")).read(),
+           "head": np.fromfile(os.path.join(REF_TOY, "head.bin"), np.int32).tolist(),
+           "out_degree": np.fromfile(os.path.join(REF_TOY, "out_degree.bin"), np.int32).tolist()}
     with open(os.path.join(HERE, "toy.json"), "w") as f:
         json.dump(toy, f, indent=1)
     print("toy", toy["canonical"], toy["ref_raw_scc_id"])
+    if a.toy_only:
+        return 0
 
     # ---- hand graphs
     pats = {}

@@ -257,3 +257,86 @@ def test_cli_drop_in(tmp_path):
     avg = O.parse_ref_average(r.stdout)
     assert set(avg) >= {"Trim", "Elephant SCC", "Mice SCC", "Total time", "FW BFS", "BW BFS", "Wcc", "Mice fw-bw"}
     assert G.sha(np.fromfile(out, np.int32)) == d["labels_sha256"]
+
+
+# ------------------------------------------------ text edge-list converter
+
+def test_convert_toy_matches_reference_files(solver, tmp_path):
+    """Faithful conversion of graph_converter/toy_edge_list.txt reproduces the
+    reference converter's shipped fw_begin.bin, fw_adjacent.bin, head.bin and
+    out_degree.bin byte for byte."""
+    g, d = G.toy_graph()
+    txt = tmp_path / "toy_edge_list.txt"
+    txt.write_text(d["edge_list_text"])
+    V, E = solver.convert_edge_list(str(txt), str(tmp_path), clean=False)
+    assert (V, E) == (7, 16)
+    rd = lambda n: np.fromfile(tmp_path / n, np.int32).tolist()
+    assert rd("fw_begin.bin") == d["fw_begin"] and rd("fw_adjacent.bin") == d["fw_adjacent"]
+    assert rd("head.bin") == d["head"] and rd("out_degree.bin") == d["out_degree"]
+    assert rd("bw_begin.bin") == d["bw_begin"] and rd("bw_adjacent.bin") == d["bw_adjacent"]
+    assert rd("in_degree.bin") == np.diff(d["bw_begin"]).tolist()
+    hg = I.graph_load(*[str(tmp_path / n) for n in ("fw_begin.bin", "fw_adjacent.bin", "bw_begin.bin",
+                                                      "bw_adjacent.bin")])
+    solver.upload(hg)
+    solver.solve()
+    assert solver.labels().tolist() == d["canonical"]
+
+
+def test_convert_order_duplicates_and_clean(solver, tmp_path):
+    rng = np.random.default_rng(5)
+    V0, m = 300, 2000
+    e = rng.integers(0, V0, size=(m, 2)) + 17  # ids shifted by the smallest id
+    e = np.concatenate([e, e[:100], np.array([[20, 20], [21, 21]])])
+    txt = tmp_path / "edges.txt"
+    txt.write_text("% header\n" + "\n".join(f"{a} {b}" for a, b in e) + "\n")
+    lo = int(e.min())
+    src, dst = (e[:, 0] - lo).astype(np.int32), (e[:, 1] - lo).astype(np.int32)
+    V = int(e.max()) - lo + 1
+    raw_dir, clean_dir = tmp_path / "raw", tmp_path / "clean"
+    raw_dir.mkdir()
+    clean_dir.mkdir()
+    Vr, Er = solver.convert_edge_list(str(txt), str(raw_dir), clean=False)
+    assert (Vr, Er) == (V, e.shape[0])
+    fb = np.fromfile(raw_dir / "fw_begin.bin", np.int32)
+    fa = np.fromfile(raw_dir / "fw_adjacent.bin", np.int32)
+    order = np.argsort(src, kind="stable")  # rows in input order
+    assert np.array_equal(fa, dst[order])
+    assert np.array_equal(np.diff(fb), np.bincount(src, minlength=V))
+    bb = np.fromfile(raw_dir / "bw_begin.bin", np.int32)
+    ba = np.fromfile(raw_dir / "bw_adjacent.bin", np.int32)
+    assert np.array_equal(ba, src[np.argsort(dst, kind="stable")])
+    Vc, Ec = solver.convert_edge_list(str(txt), str(clean_dir), clean=True)
+    og = O.build_csr(V, src, dst)
+    assert Ec == og.E
+    for name, ref in (("fw_begin.bin", og.fw_beg), ("fw_adjacent.bin", og.fw_adj), ("bw_begin.bin", og.bw_beg),
+                      ("bw_adjacent.bin", og.bw_adj)):
+        assert np.array_equal(np.fromfile(clean_dir / name, np.int32), ref), name
+    # the raw (duplicated, self-looped, unsorted) files solve to the same labels
+    hg = I.graph_load(*[str(raw_dir / n) for n in ("fw_begin.bin", "fw_adjacent.bin", "bw_begin.bin",
+                                                     "bw_adjacent.bin")])
+    solver.upload(hg)
+    solver.solve()
+    assert np.array_equal(solver.labels(), O.tarjan(og))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 2\n3\n")
+    with pytest.raises(I.IspanError) as ex:
+        solver.convert_edge_list(str(bad), str(tmp_path))
+    assert ex.value.code == -7
+
+
+# ---------------------------------------------- both giant-phase engines
+
+@pytest.mark.parametrize("engine", [0, 1])
+@pytest.mark.parametrize("seed", range(16))
+def test_engines_random(solver, engine, seed):
+    g = G.random_graph(500 + seed, vmax=[60, 800, 5000][seed % 3], density=[0.7, 2.0, 5.0][seed % 3])
+    check(solver, g, engine=engine, **PARAMS[seed % len(PARAMS)])
+
+
+@pytest.mark.parametrize("engine", [0, 1])
+def test_engines_structured(solver, engine):
+    for g in (O.rmat(17, 16, 2), O.rmat(15, 3, 4), O.grid(300, 0.5, 6), G.cycles_graph(3, n_cycles=400)):
+        check(solver, g, engine=engine)
+    for name, d in G.pattern_cases():
+        lab, _st = solve(solver, G.pattern_graph(d), engine=engine)
+        assert lab.tolist() == d["canonical"], name
