@@ -108,6 +108,7 @@ struct GiantArgs {
     int32_t* stamp;           // pattern size per labelled vertex
     int32_t* parent;          // WCC union-find forest
     int32_t* fq_n_ctl;        // Ctl::fq_n: receives the final queue length at [final buffer]
+    int32_t* dbg;             // 2 x grid: barriers each block arrived at / passed
     int32_t alpha, beta, gamma;
     double avg_degree;
     int32_t wrap32;

@@ -21,6 +21,13 @@ __device__ __forceinline__ T load_agent(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Device-coherent store: written through to the device coherence point, so
+// a block on another XCD that has invalidated its caches reads it.
+template <class T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Lanes of one wave exchange data through memory: order and publish it.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -69,22 +76,29 @@ struct WaveQueue {
     int n;         // wave-uniform fill
 };
 
+// COH: entries are written device-coherently (read by other blocks of the
+// same launch after a grid barrier).
+template <bool COH = false>
 __device__ __forceinline__ void wq_flush(WaveQueue& wq, int32_t* gq, int32_t* gcount) {
     if (wq.n == 0) return;
     wave_sync();
     int base = 0;
     if (lane_id() == 0) base = atomicAdd(gcount, wq.n);
     base = __shfl(base, 0, kWave);
-    for (int i = lane_id(); i < wq.n; i += kWave) gq[base + i] = wq.buf[i];
+    for (int i = lane_id(); i < wq.n; i += kWave) {
+        if (COH) st_agent(&gq[base + i], wq.buf[i]);
+        else gq[base + i] = wq.buf[i];
+    }
     wq.n = 0;
     wave_sync();
 }
 
+template <bool COH = false>
 __device__ __forceinline__ void wq_push(WaveQueue& wq, bool pred, int32_t v, int32_t* gq, int32_t* gcount) {
     uint64_t m = __ballot(pred);
     if (m == 0) return;
     int c = __popcll(m);
-    if (wq.n + c > kWaveBuf) wq_flush(wq, gq, gcount);
+    if (wq.n + c > kWaveBuf) wq_flush<COH>(wq, gq, gcount);
     if (pred) wq.buf[wq.n + __popcll(m & lanemask_lt())] = v;
     wq.n += c;
 }
@@ -163,34 +177,38 @@ __device__ __forceinline__ bool pivot_better(int64_t p, int32_t i, int64_t bp, i
 // detected group is an SCC of the active subgraph, so two detections of one
 // vertex always agree on its label.
 
-// Distinct active neighbours of x other than x, e1, e2: returns how many
-// were seen, stopping at cap + 1; the first (up to 2) are in g0, g1.
-__device__ __forceinline__ int nb_scan(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
-                                       const uint64_t* __restrict__ act, int32_t x, int32_t e1, int32_t e2, int cap,
-                                       int32_t& g0, int32_t& g1) {
-    int n = 0;
+// Distinct active neighbours of x other than x, e1, e2: how many were seen
+// (stopping at cap + 1) and the first two of them.
+struct Nb {
+    int n;
+    int32_t g0, g1;
+};
+
+__device__ __forceinline__ Nb nb_scan(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
+                                      const uint64_t* __restrict__ act, int32_t x, int32_t e1, int32_t e2, int cap) {
+    Nb r{0, -1, -1};
     const int32_t end = beg[x + 1];
     for (int32_t k = beg[x]; k < end; ++k) {
         const int32_t w = adj[k];
         if (w == x || w == e1 || w == e2 || !bit_of(act, w)) continue;
-        if ((n >= 1 && w == g0) || (n >= 2 && w == g1)) continue;
-        if (n == 0) g0 = w;
-        else if (n == 1) g1 = w;
-        if (++n > cap) break;
+        if ((r.n >= 1 && w == r.g0) || (r.n >= 2 && w == r.g1)) continue;
+        if (r.n == 0) r.g0 = w;
+        else if (r.n == 1) r.g1 = w;
+        if (++r.n > cap) break;
     }
-    return n;
+    return r;
 }
 
 __device__ __forceinline__ void pattern_mark(int32_t* lab, int32_t* stamp, int32_t a, int32_t b, int32_t c, int size) {
     int32_t m = min(a, b);
     if (size == 3) m = min(m, c);
-    lab[a] = m;
-    lab[b] = m;
-    stamp[a] = size;
-    stamp[b] = size;
+    st_agent(&lab[a], m);
+    st_agent(&lab[b], m);
+    st_agent(&stamp[a], size);
+    st_agent(&stamp[b], size);
     if (size == 3) {
-        lab[c] = m;
-        stamp[c] = size;
+        st_agent(&lab[c], m);
+        st_agent(&stamp[c], size);
     }
 }
 
@@ -203,30 +221,35 @@ __device__ __forceinline__ void pattern_check(const int32_t* __restrict__ fwb, c
     for (int dir = 0; dir < 2; ++dir) {
         const int32_t* beg = dir == 0 ? fwb : bwb;
         const int32_t* adj = dir == 0 ? fwa : bwa;
-        int32_t g0 = -1, g1 = -1, h0 = -1, h1 = -1, k0 = -1, k1 = -1;
-        const int na = nb_scan(beg, adj, act, a, -1, -1, 2, g0, g1);
-        if (na == 1) {
-            const int32_t b = g0;
+        const Nb na = nb_scan(beg, adj, act, a, -1, -1, 2);
+        if (na.n == 1) {
+            const int32_t b = na.g0;
             // size 2: b's only other one is a
-            if (nb_scan(beg, adj, act, b, -1, -1, 1, h0, h1) == 1 && h0 == a) {
+            const Nb nb = nb_scan(beg, adj, act, b, -1, -1, 1);
+            if (nb.n == 1 && nb.g0 == a) {
                 pattern_mark(lab, stamp, a, b, -1, 2);
                 return;
             }
             // size 3 cycle: a -> b -> c -> a
-            if (nb_scan(beg, adj, act, b, a, -1, 1, h0, h1) == 1) {
-                const int32_t c = h0;
-                if (nb_scan(beg, adj, act, c, b, -1, 1, k0, k1) == 1 && k0 == a) {
+            const Nb nb2 = nb_scan(beg, adj, act, b, a, -1, 1);
+            if (nb2.n == 1) {
+                const int32_t c = nb2.g0;
+                const Nb nc = nb_scan(beg, adj, act, c, b, -1, 1);
+                if (nc.n == 1 && nc.g0 == a) {
                     pattern_mark(lab, stamp, a, b, c, 3);
                     return;
                 }
             }
-        } else if (na == 2) {
+        } else if (na.n == 2) {
             // size 3 hub: a <-> b, a <-> c
-            const int32_t b = g0, c = g1;
-            if (nb_scan(beg, adj, act, b, b, c, 1, h0, h1) == 1 && h0 == a &&
-                nb_scan(beg, adj, act, c, b, c, 1, k0, k1) == 1 && k0 == a) {
-                pattern_mark(lab, stamp, a, b, c, 3);
-                return;
+            const int32_t b = na.g0, c = na.g1;
+            const Nb nb = nb_scan(beg, adj, act, b, b, c, 1);
+            if (nb.n == 1 && nb.g0 == a) {
+                const Nb nc = nb_scan(beg, adj, act, c, b, c, 1);
+                if (nc.n == 1 && nc.g0 == a) {
+                    pattern_mark(lab, stamp, a, b, c, 3);
+                    return;
+                }
             }
         }
     }

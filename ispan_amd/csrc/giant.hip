@@ -39,7 +39,7 @@ struct Smem {
 };
 
 struct Ctx {
-    GiantArgs a;
+    const GiantArgs& a;
     int G, b, tid, lane, wave;
     Smem& sm;
     int64_t edges_td, edges_bu, found_td, found_bu;  // work counters (thread 0's are the block's)
@@ -47,27 +47,34 @@ struct Ctx {
 
 // Two-level grid barrier: blocks arrive on one of 32 counters, the last of
 // each group arrives on the top counter, the last overall bumps `gen`.
-// Returns false (and flags an error) if a wait exceeds kSpinCap.
-__device__ bool gsync(GiantState* gs, int G, Smem& sm) {
+// Memory protocol: everything a block writes for other blocks is written
+// device-coherently (atomics or st_agent), so arriving needs no cache
+// writeback -- only that the block's own accesses have completed
+// (workgroup-scope fence = s_waitcnt).  After passing, one device-scope
+// acquire per block invalidates its caches, so plain loads see what other
+// blocks wrote before the barrier.  Returns false (and flags an error) if a
+// wait exceeds kSpinCap.
+__device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
+        dbg[blockIdx.x] += 1;  // barriers arrived at
         const int32_t g0 = load_agent(&gs->gen.v);
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         const int grp = blockIdx.x % kBarGroups;
         const int ngrp = min(kBarGroups, G);
         const int gsize = (G - grp + kBarGroups - 1) / kBarGroups;
         bool last = false;
         if (atomicAdd(&gs->sub[grp].v, 1) == gsize - 1) {
-            __hip_atomic_store(&gs->sub[grp].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
+            st_agent(&gs->sub[grp].v, 0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (atomicAdd(&gs->top.v, 1) == ngrp - 1) {
-                __hip_atomic_store(&gs->top.v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                st_agent(&gs->top.v, 0);
                 last = true;
             }
         }
         if (last) {
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             atomicAdd(&gs->gen.v, 1);
         } else {
             int spins = 0;
@@ -80,15 +87,16 @@ __device__ bool gsync(GiantState* gs, int G, Smem& sm) {
                 }
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         sm.ok = ok && load_agent(&gs->err.v) == 0;
+        if (ok) dbg[G + blockIdx.x] += 1;  // barriers passed
     }
     __syncthreads();
     return sm.ok;
 }
 
 // Sum of a per-thread value over the block (every thread gets it).
-__device__ int64_t block_sum(int64_t x, Smem& sm) {
+__device__ __forceinline__ int64_t block_sum(int64_t x, Smem& sm) {
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
     if (lane_id() == 0) sm.red[wave_id()] = x;
@@ -100,13 +108,13 @@ __device__ int64_t block_sum(int64_t x, Smem& sm) {
 
 // Sum / minimum of one partial slot over all blocks (every block computes
 // the same value).
-__device__ int64_t sum_slot(const int64_t* part, int slot, int G, Smem& sm) {
+__device__ __forceinline__ int64_t sum_slot(const int64_t* part, int slot, int G, Smem& sm) {
     int64_t x = 0;
     for (int i = threadIdx.x; i < G; i += kBlock) x += load_agent(&part[(int64_t)slot * G + i]);
     return block_sum(x, sm);
 }
 
-__device__ int64_t min_slot(const int64_t* part, int slot, int G, Smem& sm) {
+__device__ __forceinline__ int64_t min_slot(const int64_t* part, int slot, int G, Smem& sm) {
     int64_t x = INT64_MAX;
     for (int i = threadIdx.x; i < G; i += kBlock) x = min(x, load_agent(&part[(int64_t)slot * G + i]));
 #pragma unroll
@@ -130,17 +138,20 @@ __device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* 
         if (d >= kGiantHubDeg) {
             const int nch = (d + kGiantChunk - 1) / kGiantChunk;
             const int h = atomicAdd(hn_out, nch);
-            for (int c = 0; c < nch; ++c) hout[h + c] = HubItem{w, s0 + c * kGiantChunk};
+            for (int c = 0; c < nch; ++c) {
+                st_agent(&hout[h + c].v, w);
+                st_agent(&hout[h + c].start, s0 + c * kGiantChunk);
+            }
         }
     }
-    wq_push(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
+    wq_push<true>(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
 }
 
 // One direction of the traversal.  Returns the number of vertices claimed
 // (the root excluded), or -1 after a failed barrier.  BW: claims are limited
 // to the forward set and *gmin receives the smallest id reached.
 template <bool BW>
-__device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* levels_out, int32_t* gmin_out) {
+__device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* levels_out, int32_t* gmin_out) {
     const GiantArgs& a = c.a;
     Smem& sm = c.sm;
     const int32_t* beg = BW ? a.bwb : a.fwb;  // expansion (top-down) direction
@@ -252,7 +263,7 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
                     }
                 }
             }
-            wq_flush(wq, qout, qn_out);
+            wq_flush<true>(wq, qout, qn_out);
         } else {
             // ---- bottom-up over the candidate bitmap, one wave per 64-vertex
             // word; only that wave writes the word here.  Its visited bits
@@ -262,7 +273,7 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
             for (int64_t w = gwave; w < a.nwords; w += total_waves) {
                 const uint64_t vw = load_agent(&vis[w]);
                 const uint64_t cand = load_agent(&allow[w]) & ~vw;
-                if (c.lane == 0) a.snap[w] = vw;
+                if (c.lane == 0) st_agent(&a.snap[w], vw);
                 if (!cand) continue;
                 const int32_t v = (int32_t)(w * 64 + c.lane);
                 const bool is_c = (cand >> c.lane) & 1ull;
@@ -279,7 +290,8 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
                 }
                 found = wave_any_edge(is_c && !found, p, e, v, radj, is_vis, scanned) || found;
                 const uint64_t fm = __ballot(found);
-                if (fm && c.lane == 0) vis[w] = vw | fm;
+                if (fm && c.lane == 0)
+                    atomicOr(reinterpret_cast<unsigned long long*>(vis + w), (unsigned long long)fm);
                 my_found += found;
                 if (BW && found) gmin = min(gmin, v);
             }
@@ -297,10 +309,10 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
             __syncthreads();
         }
         if (c.tid == 0) {
-            a.part[(int64_t)fslot * c.G + c.b] = blk;
-            if (BW) a.part[(int64_t)mslot * c.G + c.b] = min(min(sm.tmp[0], sm.tmp[1]), min(sm.tmp[2], sm.tmp[3]));
+            st_agent(&a.part[(int64_t)fslot * c.G + c.b], (int64_t)(blk));
+            if (BW) st_agent(&a.part[(int64_t)mslot * c.G + c.b], (int64_t)(min(min(sm.tmp[0], sm.tmp[1]), min(sm.tmp[2], sm.tmp[3]))));
         }
-        if (!gsync(gs, c.G, sm)) return -1;
+        if (!gsync(gs, c.G, sm, a.dbg)) return -1;
         const int64_t found = sum_slot(a.part, fslot, c.G, sm);
         if (BW) *gmin_out = (int32_t)min_slot(a.part, mslot, c.G, sm);
         visited += found;
@@ -325,9 +337,9 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
                 const int32_t v = (int32_t)(w * 64 + c.lane);
                 push_next((nw >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, h_next, hn_next);
             }
-            wq_flush(wq, q_next, qn_next);
+            wq_flush<true>(wq, q_next, qn_next);
             mode = 2;
-            if (!gsync(gs, c.G, sm)) return -1;
+            if (!gsync(gs, c.G, sm, a.dbg)) return -1;
         }
     }
     *levels_out = L;
@@ -339,7 +351,7 @@ __device__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* lev
 // One trim-1 sweep (trim_1_from_fq_gfq, src/trim_1_gfq.h:365-458) over
 // fin[0, n): act is updated in place, survivors are pushed to fout.
 // Returns the number trimmed (block partial written to `slot`).
-__device__ void trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount, int slot) {
+__device__ __forceinline__ void trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount, int slot) {
     const GiantArgs& a = c.a;
     WaveQueue wq{c.sm.wbuf[c.wave], 0};
     const uint64_t* act = a.act;
@@ -358,18 +370,18 @@ __device__ void trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout,
             wave_any_edge(alive && out, alive ? a.bwb[v] : 0, alive ? a.bwb[v + 1] : 0, v, a.bwa, live, scanned);
         const bool trim = alive && !(out && in);
         if (trim) {
-            a.lab[v] = v;
+            st_agent(&a.lab[v], v);
             atomicAnd(reinterpret_cast<unsigned long long*>(a.act + (v >> 6)), ~(1ull << (v & 63)));
             ++trimmed;
         }
-        wq_push(wq, alive && !trim, v, fout, fcount);
+        wq_push<true>(wq, alive && !trim, v, fout, fcount);
     }
-    wq_flush(wq, fout, fcount);
+    wq_flush<true>(wq, fout, fcount);
     const int64_t t = block_sum(trimmed, c.sm);
-    if (c.tid == 0) a.part[(int64_t)slot * c.G + c.b] = t;
+    if (c.tid == 0) st_agent(&a.part[(int64_t)slot * c.G + c.b], (int64_t)(t));
 }
 
-__device__ int32_t uf_root_g(int32_t* parent, int32_t v) {
+__device__ __forceinline__ int32_t uf_root_g(int32_t* parent, int32_t v) {
     int32_t nx = load_agent(&parent[v]);
     while (nx != v) {
         v = nx;
@@ -379,21 +391,24 @@ __device__ int32_t uf_root_g(int32_t* parent, int32_t v) {
 }
 
 // Trims and WCC after the giant.  Returns false after a failed barrier.
-__device__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
+__device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
     const GiantArgs& a = c.a;
     GiantState* gs = a.gs;
-    int32_t* buf[2] = {a.fq_alt, a.fq};
     int cur = 1, s = 0;
+    auto buf = [&a](int k) { return k ? a.fq : a.fq_alt; };
     int64_t prev = fq0, trim1 = 0, p2 = 0, p3 = 0;
     int it = 0;
     const int64_t nthreads = (int64_t)c.G * kBlock;
     const int64_t gt = (int64_t)c.b * kBlock + c.tid;
+    // Sweep s pushes into fqn[s % 3]; all blocks read it after the barrier.
+    // It clears fqn[(s + 1) % 3] for the next sweep: that counter was last read
+    // after sweep s - 2, i.e. before the barrier that ended sweep s - 1.
     auto sweep = [&]() -> bool {
         if (c.b == 0 && c.tid == 0)
-            __hip_atomic_store(&gs->fqn[(s + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gs->fqn[(s + 1) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int slot = (s & 1) ? kTrim1 : kTrim0;
-        trim_sweep(c, buf[cur], n, buf[cur ^ 1], &gs->fqn[s % 3].v, slot);
-        if (!gsync(gs, c.G, c.sm)) return false;
+        trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v, slot);
+        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         trim1 += sum_slot(a.part, slot, c.G, c.sm);
         n = load_agent(&gs->fqn[s % 3].v);
         cur ^= 1;
@@ -411,12 +426,12 @@ __device__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
     bool wcc = false;
     if (n > 0) {
         // P8-P10: pattern detection against the current act, then commit
-        const int32_t* f = buf[cur];
+        const int32_t* f = buf(cur);
         for (int64_t i = gt; i < n; i += nthreads) {
             const int32_t v = load_agent(&f[i]);
             if (bit_of(a.act, v)) pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v);
         }
-        if (!gsync(gs, c.G, c.sm)) return false;
+        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         int c2 = 0, c3 = 0;
         for (int64_t i = gt; i < n; i += nthreads) {
             const int32_t v = load_agent(&f[i]);
@@ -428,27 +443,27 @@ __device__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
         }
         const int64_t b2 = block_sum(c2, c.sm), b3 = block_sum(c3, c.sm);
         if (c.tid == 0) {
-            a.part[(int64_t)kPat2 * c.G + c.b] = b2;
-            a.part[(int64_t)kPat3 * c.G + c.b] = b3;
+            st_agent(&a.part[(int64_t)kPat2 * c.G + c.b], (int64_t)(b2));
+            st_agent(&a.part[(int64_t)kPat3 * c.G + c.b], (int64_t)(b3));
         }
-        if (!gsync(gs, c.G, c.sm)) return false;
+        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         p2 = sum_slot(a.part, kPat2, c.G, c.sm);
         p3 = sum_slot(a.part, kPat3, c.G, c.sm);
         // P11: one more trim-1 sweep
         if (!sweep()) return false;
         if (n > 0) {
             // P12: WCC by union-find; parent = smallest id of the WCC
-            const int32_t* g = buf[cur];
+            const int32_t* g = buf(cur);
             for (int64_t i = gt; i < n; i += nthreads) {
                 const int32_t v = load_agent(&g[i]);
-                a.parent[v] = v;
+                st_agent(&a.parent[v], v);
             }
-            if (!gsync(gs, c.G, c.sm)) return false;
+            if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
             for (int64_t i = gt; i < n; i += nthreads) {
                 const int32_t v = load_agent(&g[i]);
                 for (int32_t k = a.fwb[v]; k < a.fwb[v + 1]; ++k) {
                     const int32_t w = a.fwa[k];
-                    if (w == v || !bit_of(a.act, w)) continue;
+                    if (w == v || !((load_agent(&a.act[w >> 6]) >> (w & 63)) & 1ull)) continue;
                     int32_t x = v, y = w;
                     while (true) {
                         x = uf_root_g(a.parent, x);
@@ -463,10 +478,10 @@ __device__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
                     }
                 }
             }
-            if (!gsync(gs, c.G, c.sm)) return false;
+            if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
             for (int64_t i = gt; i < n; i += nthreads) {
                 const int32_t v = load_agent(&g[i]);
-                a.parent[v] = uf_root_g(a.parent, v);
+                st_agent(&a.parent[v], uf_root_g(a.parent, v));
             }
             wcc = true;
         }
@@ -486,8 +501,12 @@ __device__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
 
 }  // namespace
 
-__global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
+__global__ __launch_bounds__(kBlock) void k_giant(const GiantArgs a) {
     __shared__ Smem sm;
+    if (threadIdx.x == 0) {
+        a.dbg[blockIdx.x] = 0;
+        a.dbg[gridDim.x + blockIdx.x] = 0;
+    }
     Ctx c{a, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, lane_id(), wave_id(), sm, 0, 0, 0, 0};
     GiantState* gs = a.gs;
     const int64_t total_waves = (int64_t)c.G * kW;
@@ -516,9 +535,9 @@ __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
         }
         const uint64_t m = __ballot(s);
         if (c.lane == 0) {
-            a.act[w] = m;
-            a.fvis[w] = 0;
-            a.bvis[w] = 0;
+            st_agent(&a.act[w], (uint64_t)m);
+            st_agent(&a.fvis[w], (uint64_t)0);
+            st_agent(&a.bvis[w], (uint64_t)0);
             surv += __popcll(m);
         }
     }
@@ -542,11 +561,11 @@ __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
                 bestp = sm.red2[w];
                 besti = sm.bi[w];
             }
-        a.part[(int64_t)kPivP * c.G + c.b] = bestp;
-        a.part[(int64_t)kPivI * c.G + c.b] = besti;
-        a.part[(int64_t)kSurv * c.G + c.b] = surv_blk;
+        st_agent(&a.part[(int64_t)kPivP * c.G + c.b], (int64_t)(bestp));
+        st_agent(&a.part[(int64_t)kPivI * c.G + c.b], (int64_t)(besti));
+        st_agent(&a.part[(int64_t)kSurv * c.G + c.b], (int64_t)(surv_blk));
     }
-    if (!gsync(gs, c.G, sm)) return;
+    if (!gsync(gs, c.G, sm, a.dbg)) return;
     // every block reduces the pivot candidates and the survivor count
     bestp = 0;
     besti = INT32_MAX;
@@ -607,22 +626,22 @@ __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
         const int64_t v = w * 64 + c.lane;
         if (v < a.V) {
             const bool gi = (g >> c.lane) & 1ull, ac = (a0 >> c.lane) & 1ull;
-            a.lab[v] = gi ? gmin : (ac ? -1 : (int32_t)v);
+            st_agent(&a.lab[v], gi ? gmin : (ac ? -1 : (int32_t)v));
         }
         if (c.lane == 0) {
-            if (g) a.act[w] = a0 & ~g;
+            if (g) st_agent(&a.act[w], a0 & ~g);
             cnt += __popcll(a0 & ~g);
         }
     }
     const int64_t cnt_blk = block_sum(cnt, sm);
     if (c.tid == 0) {
-        a.part[(int64_t)kPost * c.G + c.b] = cnt_blk;
-        a.part[(int64_t)kEdgeTD * c.G + c.b] = c.edges_td;
-        a.part[(int64_t)kEdgeBU * c.G + c.b] = c.edges_bu;
-        a.part[(int64_t)kFoundTD * c.G + c.b] = c.found_td;
-        a.part[(int64_t)kFoundBU * c.G + c.b] = c.found_bu;
+        st_agent(&a.part[(int64_t)kPost * c.G + c.b], (int64_t)(cnt_blk));
+        st_agent(&a.part[(int64_t)kEdgeTD * c.G + c.b], (int64_t)(c.edges_td));
+        st_agent(&a.part[(int64_t)kEdgeBU * c.G + c.b], (int64_t)(c.edges_bu));
+        st_agent(&a.part[(int64_t)kFoundTD * c.G + c.b], (int64_t)(c.found_td));
+        st_agent(&a.part[(int64_t)kFoundBU * c.G + c.b], (int64_t)(c.found_bu));
     }
-    if (!gsync(gs, c.G, sm)) return;
+    if (!gsync(gs, c.G, sm, a.dbg)) return;
     // exclusive prefix of the block counts, then an ordered block-local scan
     int64_t off, tot;
     {
@@ -637,14 +656,14 @@ __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
     }
     for (int64_t base = wb; base < we; base += kBlock) {
         const int64_t w = base + c.tid;
-        const uint64_t m = w < we ? a.act[w] : 0ull;
+        const uint64_t m = w < we ? load_agent(&a.act[w]) : 0ull;
         int excl, total;
         block_excl_scan(__popcll(m), excl, total, sm.tmp);
         uint64_t mm = m;
         int64_t o = off + excl;
         while (mm) {
             const int bit = __ffsll((unsigned long long)mm) - 1;
-            a.fq[o++] = (int32_t)(w * 64 + bit);
+            st_agent(&a.fq[o++], (int32_t)(w * 64 + bit));
             mm &= mm - 1;
         }
         off += total;
@@ -670,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_giant(GiantArgs a) {
         }
     }
     if (a.rest) {
-        if (!gsync(gs, c.G, sm)) return;  // queue written by every block
+        if (!gsync(gs, c.G, sm, a.dbg)) return;  // queue written by every block
         if (!rest_phase(c, fq0, tot)) return;
         if (stamp) gs->ts[5] = wall_clock64();
     }

@@ -70,6 +70,7 @@ struct ispan_ctx {
     uint64_t* snap = nullptr;
     HubItem* gh[2] = {nullptr, nullptr};
     int64_t* gpart = nullptr;
+    int32_t* gdbg = nullptr;
     GiantState* gstate = nullptr;
     GiantState* h_gstate = nullptr;  // pinned
     hipEvent_t ev[16] = {};
@@ -131,6 +132,7 @@ void free_solve_buffers(ispan_ctx* ctx) {
     dfree(ctx->gh[0]);
     dfree(ctx->gh[1]);
     dfree(ctx->gpart);
+    dfree(ctx->gdbg);
     if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
     ctx->sort_tmp = nullptr;
     ctx->sort_tmp_bytes = 0;
@@ -175,6 +177,7 @@ int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
     CK(hipMalloc(&ctx->snap, nw * 8));
     for (int i = 0; i < 2; ++i) CK(hipMalloc(&ctx->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem)));
     CK(hipMalloc(&ctx->gpart, (size_t)kGiantSlots * std::max(ctx->coop_grid, 1) * 8));
+    CK(hipMalloc(&ctx->gdbg, (size_t)2 * std::max(ctx->coop_grid, 1) * 4));
     size_t bytes = 0;
     CK(rocprim::radix_sort_keys(nullptr, bytes, ctx->keys, ctx->keys_alt, (size_t)nv, 0, 64, ctx->stream));
     CK(hipMalloc(&ctx->sort_tmp, bytes + 256));
@@ -388,6 +391,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.stamp = ctx->stamp;
         ga.parent = ctx->parent;
         ga.fq_n_ctl = ctx->ctl->fq_n;
+        ga.dbg = ctx->gdbg;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
         if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) {
             ctx->recs.push_back({3, ga0, ga1});
@@ -403,6 +407,21 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         if ((rc = readback(ctx, &syncs))) return rc;
         const GiantState& gs = *ctx->h_gstate;
         if (gs.err.v) {
+            if (gs.err.v & 1) {  // where did the blocks stop?
+                std::vector<int32_t> d((size_t)2 * ctx->coop_grid);
+                if (hipMemcpy(d.data(), ctx->gdbg, d.size() * 4, hipMemcpyDeviceToHost) == hipSuccess) {
+                    int amin = INT32_MAX, amax = 0, pmin = INT32_MAX, pmax = 0, zero = 0;
+                    for (int b = 0; b < ctx->coop_grid; ++b) {
+                        amin = std::min(amin, d[b]);
+                        amax = std::max(amax, d[b]);
+                        pmin = std::min(pmin, d[ctx->coop_grid + b]);
+                        pmax = std::max(pmax, d[ctx->coop_grid + b]);
+                        zero += d[b] == 0;
+                    }
+                    fprintf(stderr, "[ispan] barrier timeout: grid %d, arrived min %d max %d (blocks at 0: %d), "
+                            "passed min %d max %d, gen %d\n", ctx->coop_grid, amin, amax, zero, pmin, pmax, gs.gen.v);
+                }
+            }
             ctx->err = gs.err.v & 1   ? "giant phase: grid barrier timed out"
                        : gs.err.v & 2 ? "giant phase: level cap exceeded"
                                       : "giant phase: queue count out of range";
@@ -712,7 +731,8 @@ int ispan_ctx_create(ispan_ctx** out, int device) {
         (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
         (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device);
         ctx->clock_khz = rate;
-        const int per_cu = coop ? std::min(giant_max_blocks_per_cu(), 8) : 0;
+        int per_cu = coop ? std::min(giant_max_blocks_per_cu(), 8) : 0;
+        if (const char* env = getenv("ISPAN_COOP_BLOCKS_PER_CU")) per_cu = std::min(per_cu, std::max(atoi(env), 1));
         ctx->coop_grid = per_cu * ctx->cu;
     }
     if (hipMalloc(&ctx->gstate, sizeof(GiantState)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
