@@ -123,10 +123,8 @@ def main(argv=None) -> int:
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
         return 2
 
-    import torch
     import ispan_amd as I
 
-    torch.cuda.init()
     S = I.Solver(0)
     t0 = time.time()
     dg, cfg = build_graph(S, a.workload, a.seed)
@@ -147,7 +145,7 @@ def main(argv=None) -> int:
     launches = [0] * 4
     edges = [0] * 4
     found = [0] * 4
-    torch.cuda.synchronize()
+    S.synchronize()
     t_start = time.perf_counter()
     for _ in range(a.steps):
         S.solve(p_timed)
@@ -158,7 +156,7 @@ def main(argv=None) -> int:
                 launches[c] += kp.launches[c]
                 edges[c] += kp.edges[c]
                 found[c] += kp.found[c]
-    torch.cuda.synchronize()
+    S.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed * 1e3 / a.steps
     labels = S.labels()

@@ -106,6 +106,9 @@ int ispan_ctx_destroy(ispan_ctx* ctx);
 /* Use a caller-owned hipStream_t (passed as void*) for all work; NULL = the
  * context's own stream. */
 int ispan_ctx_set_stream(ispan_ctx* ctx, void* hip_stream);
+/* Wait until all work queued by this context has finished (the reference's
+ * calls are synchronous; this is the point where its timer would stop). */
+int ispan_ctx_synchronize(ispan_ctx* ctx);
 /* Message of the last error seen by this context. */
 const char* ispan_ctx_last_error(const ispan_ctx* ctx);
 

@@ -32,7 +32,7 @@ ERRORS = {-1: "ISPAN_ERR_ARG", -2: "ISPAN_ERR_CSR", -3: "ISPAN_ERR_NOMEM", -4: "
 # Every symbol include/ispan_hip.h declares.
 EXPORTS = [
     "ispan_default_params", "ispan_strerror", "ispan_ctx_create", "ispan_ctx_destroy", "ispan_ctx_set_stream",
-    "ispan_ctx_last_error", "ispan_graph_upload", "ispan_graph_bind_device", "ispan_solve", "ispan_labels_download",
+    "ispan_ctx_synchronize", "ispan_ctx_last_error", "ispan_graph_upload", "ispan_graph_bind_device", "ispan_solve", "ispan_labels_download",
     "ispan_labels_device", "ispan_solve_stats", "ispan_scc_detection", "ispan_graph_load", "ispan_graph_free",
     "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
     "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
@@ -122,6 +122,7 @@ def lib():
         "ispan_ctx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
         "ispan_ctx_destroy": (C.c_int, [_vp]),
         "ispan_ctx_set_stream": (C.c_int, [_vp, _vp]),
+        "ispan_ctx_synchronize": (C.c_int, [_vp]),
         "ispan_ctx_last_error": (C.c_char_p, [_vp]),
         "ispan_graph_upload": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params)]),
         "ispan_graph_bind_device": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params)]),
@@ -287,6 +288,10 @@ class Solver:
 
     def set_stream(self, stream_handle: int | None):
         _check(lib().ispan_ctx_set_stream(self._ctx, stream_handle or None), self._ctx)
+
+    def synchronize(self):
+        """Wait for all work queued by this context."""
+        _check(lib().ispan_ctx_synchronize(self._ctx), self._ctx)
 
     def upload(self, g: HostGraph, params: Params | None = None):
         arrs = [np.ascontiguousarray(a, np.int32) for a in (g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)]

@@ -60,6 +60,14 @@ struct alignas(128) GLine {
     int32_t pad[31];
 };
 
+// Reduction carried by one grid barrier: two sums and a max over keys
+// (identity 0 for all three).
+struct alignas(128) RLine {
+    int64_t sum[2];
+    int64_t key;
+    int64_t pad[13];
+};
+
 // Device state of one k_giant launch; zeroed before every launch.
 struct GiantState {
     GLine sub[32];      // barrier arrival counters (two-level barrier)
@@ -74,6 +82,9 @@ struct GiantState {
     GLine fqn[3];       // active-queue lengths of the in-kernel trim sweeps (rotating)
     int64_t n_final, final_sel, trim1, pat2, pat3, sweeps, wcc_done;
     int64_t pad[5];
+    RLine gacc[32];     // barrier reductions: per arrival group
+    RLine tacc;         // barrier reductions: over groups
+    RLine res[2];       // barrier reductions: result of generation g in res[g & 1]
 };
 
 struct GiantArgs {

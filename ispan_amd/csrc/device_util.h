@@ -58,14 +58,18 @@ __device__ __forceinline__ int wave_min(int x) {
 }
 
 // Block-wide exclusive scan (256 threads); every thread gets the total.
-__device__ __forceinline__ void block_excl_scan(int x, int& excl, int& total, int* s_tmp /*[4]*/) {
-    int incl = wave_incl_scan(x);
+template <int NW = 4>
+__device__ __forceinline__ void block_excl_scan(int x, int& excl, int& total, int* s_tmp /*[NW]*/) {
+    static_assert(NW <= kWave, "one lane per wave total");
+    const int incl = wave_incl_scan(x);
     if (lane_id() == kWave - 1) s_tmp[wave_id()] = incl;
     __syncthreads();
-    int w0 = s_tmp[0], w1 = s_tmp[1], w2 = s_tmp[2], w3 = s_tmp[3];
-    int wb = wave_id() == 0 ? 0 : wave_id() == 1 ? w0 : wave_id() == 2 ? w0 + w1 : w0 + w1 + w2;
-    excl = wb + incl - x;
-    total = w0 + w1 + w2 + w3;
+    // lane i of every wave scans the NW wave totals
+    const int t = lane_id() < NW ? s_tmp[lane_id()] : 0;
+    const int ti = wave_incl_scan(t);
+    const int wv = wave_id();
+    excl = __shfl(ti - t, wv, kWave) + incl - x;
+    total = __shfl(ti, NW - 1, kWave);
     __syncthreads();
 }
 

@@ -18,21 +18,22 @@ namespace {
 
 constexpr int kSpinCap = 1 << 24;  // barrier wait bound (~1 s); beyond it the kernel aborts
 constexpr int kBarGroups = 32;     // arrival counters of the two-level barrier
-constexpr int kW = kBlock / kWave;
+// One 1024-thread block per CU: every block invalidates its caches after a
+// barrier, so fewer, larger blocks make the barrier cheaper.
+constexpr int kGBlock = 1024;
+constexpr int kW = kGBlock / kWave;
 
 // Partial-count slots: part[slot * grid + block].
-enum Slot {
-    kPivP, kPivI, kSurv, kFound0, kFound1, kMin0, kMin1, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU, kTrim0, kTrim1,
-    kPat2, kPat3
-};
+enum Slot { kPivP, kPivI, kSurv, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
 static_assert(kFoundBU < kGiantSlots, "slot count");
 
 struct Smem {
-    int32_t off[kBlock];
-    int32_t start[kBlock];
+    int32_t off[kGBlock];
+    int32_t start[kGBlock];
     int tmp[kW];
     int64_t red[kW];
     int64_t red2[kW];
+    int64_t rres[3];  // reduction result of the last barrier
     int32_t bi[kW];
     int ok;
     int32_t wbuf[kW][kWaveBuf];
@@ -45,31 +46,67 @@ struct Ctx {
     int64_t edges_td, edges_bu, found_td, found_bu;  // work counters (thread 0's are the block's)
 };
 
+// Values a block contributes to the reduction a barrier carries: two sums
+// and a max over non-negative keys (0 contributes nothing).
+struct Red {
+    int64_t s0, s1, key;
+};
+
 // Two-level grid barrier: blocks arrive on one of 32 counters, the last of
 // each group arrives on the top counter, the last overall bumps `gen`.
+//
 // Memory protocol: everything a block writes for other blocks is written
 // device-coherently (atomics or st_agent), so arriving needs no cache
-// writeback -- only that the block's own accesses have completed
-// (workgroup-scope fence = s_waitcnt).  After passing, one device-scope
-// acquire per block invalidates its caches, so plain loads see what other
-// blocks wrote before the barrier.  Returns false (and flags an error) if a
-// wait exceeds kSpinCap.
-__device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg) {
+// writeback, only that the block's own accesses have completed (a
+// workgroup-scope fence waits for them).  After `gen` moves, each block
+// invalidates its L1 and its XCC's L2 (device-scope acquire), so plain loads
+// after the barrier see what every block wrote before it.
+//
+// With RED, each block's `in` is folded in on the way: into its group's
+// accumulator before arriving, the group total into the top accumulator by
+// the group's last arriver, the grand total published by the last block
+// before it bumps `gen`; every block gets the same `out`.
+//
+// Returns false (and flags an error) if a wait exceeds kSpinCap.
+template <bool RED = false>
+__device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg, Red in = Red{0, 0, 0},
+                                      Red* out = nullptr) {
+    using ull = unsigned long long;
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
         dbg[blockIdx.x] += 1;  // barriers arrived at
         const int32_t g0 = load_agent(&gs->gen.v);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         const int grp = blockIdx.x % kBarGroups;
         const int ngrp = min(kBarGroups, G);
         const int gsize = (G - grp + kBarGroups - 1) / kBarGroups;
+        RLine& ga = gs->gacc[grp];
+        if (RED) {
+            if (in.s0) atomicAdd(reinterpret_cast<ull*>(&ga.sum[0]), (ull)in.s0);
+            if (in.s1) atomicAdd(reinterpret_cast<ull*>(&ga.sum[1]), (ull)in.s1);
+            if (in.key) atomicMax(reinterpret_cast<ull*>(&ga.key), (ull)in.key);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         bool last = false;
         if (atomicAdd(&gs->sub[grp].v, 1) == gsize - 1) {
             st_agent(&gs->sub[grp].v, 0);
+            if (RED) {
+                const ull a0 = atomicExch(reinterpret_cast<ull*>(&ga.sum[0]), 0ull);
+                const ull a1 = atomicExch(reinterpret_cast<ull*>(&ga.sum[1]), 0ull);
+                const ull ak = atomicExch(reinterpret_cast<ull*>(&ga.key), 0ull);
+                if (a0) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[0]), a0);
+                if (a1) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[1]), a1);
+                if (ak) atomicMax(reinterpret_cast<ull*>(&gs->tacc.key), ak);
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (atomicAdd(&gs->top.v, 1) == ngrp - 1) {
                 st_agent(&gs->top.v, 0);
+                if (RED) {
+                    RLine& r = gs->res[(g0 + 1) & 1];
+                    st_agent(&r.sum[0], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[0]), 0ull));
+                    st_agent(&r.sum[1], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[1]), 0ull));
+                    st_agent(&r.key, (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.key), 0ull));
+                }
                 last = true;
             }
         }
@@ -87,10 +124,18 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // this CU's L1 and this XCC's L2
+        if (RED && ok) {
+            const RLine& r = gs->res[(g0 + 1) & 1];
+            sm.rres[0] = load_agent(&r.sum[0]);
+            sm.rres[1] = load_agent(&r.sum[1]);
+            sm.rres[2] = load_agent(&r.key);
+        }
         sm.ok = ok && load_agent(&gs->err.v) == 0;
         if (ok) dbg[G + blockIdx.x] += 1;  // barriers passed
     }
+    __syncthreads();
+    if (RED && out) *out = Red{sm.rres[0], sm.rres[1], sm.rres[2]};
     __syncthreads();
     return sm.ok;
 }
@@ -101,7 +146,10 @@ __device__ __forceinline__ int64_t block_sum(int64_t x, Smem& sm) {
     for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
     if (lane_id() == 0) sm.red[wave_id()] = x;
     __syncthreads();
-    x = sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3];
+    x = lane_id() < kW ? sm.red[lane_id()] : 0;
+#pragma unroll
+    for (int d = kW / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    x = __shfl(x, 0, kWave);
     __syncthreads();
     return x;
 }
@@ -110,20 +158,8 @@ __device__ __forceinline__ int64_t block_sum(int64_t x, Smem& sm) {
 // the same value).
 __device__ __forceinline__ int64_t sum_slot(const int64_t* part, int slot, int G, Smem& sm) {
     int64_t x = 0;
-    for (int i = threadIdx.x; i < G; i += kBlock) x += load_agent(&part[(int64_t)slot * G + i]);
+    for (int i = threadIdx.x; i < G; i += kGBlock) x += load_agent(&part[(int64_t)slot * G + i]);
     return block_sum(x, sm);
-}
-
-__device__ __forceinline__ int64_t min_slot(const int64_t* part, int slot, int G, Smem& sm) {
-    int64_t x = INT64_MAX;
-    for (int i = threadIdx.x; i < G; i += kBlock) x = min(x, load_agent(&part[(int64_t)slot * G + i]));
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x = min(x, (int64_t)__shfl_xor(x, d, kWave));
-    if (lane_id() == 0) sm.red[wave_id()] = x;
-    __syncthreads();
-    x = min(min(sm.red[0], sm.red[1]), min(sm.red[2], sm.red[3]));
-    __syncthreads();
-    return x;
 }
 
 // Pushes a newly claimed vertex for expansion at the next level: vertices of
@@ -203,10 +239,10 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     nh = 0;
                 }
             }
-            const int64_t ntiles = (n + kBlock - 1) / kBlock;
+            const int64_t ntiles = (n + kGBlock - 1) / kGBlock;
             for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
                 if (u < ntiles) {
-                    const int64_t i = u * kBlock + c.tid;
+                    const int64_t i = u * kGBlock + c.tid;
                     int32_t start = 0, deg = 0;
                     if (i < n) {
                         const int32_t v = load_agent(&qin[i]);
@@ -214,17 +250,17 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         deg = beg[v + 1] - start;
                     }
                     int excl, total;
-                    block_excl_scan(deg, excl, total, sm.tmp);
+                    block_excl_scan<kW>(deg, excl, total, sm.tmp);
                     sm.off[c.tid] = excl;
                     sm.start[c.tid] = start;
                     if (c.tid == 0) c.edges_td += total;
                     __syncthreads();
-                    for (int eb = 0; eb < total; eb += kBlock) {
+                    for (int eb = 0; eb < total; eb += kGBlock) {
                         const int e = eb + c.tid;
                         bool claim = false;
                         int32_t w = 0;
                         if (e < total) {
-                            int lo = 0, hi = kBlock - 1;  // last j with off[j] <= e
+                            int lo = 0, hi = kGBlock - 1;  // last j with off[j] <= e
                             while (lo < hi) {
                                 const int mid = (lo + hi + 1) >> 1;
                                 if (sm.off[mid] <= e) lo = mid;
@@ -249,7 +285,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     }
                     const int32_t e1 = min(s0 + kGiantChunk, beg[v + 1]);
                     if (c.tid == 0) c.edges_td += e1 - s0;
-                    for (int32_t eb = s0; eb < e1; eb += kBlock) {
+                    for (int32_t eb = s0; eb < e1; eb += kGBlock) {
                         const int32_t e = eb + c.tid;
                         bool claim = false;
                         int32_t w = 0;
@@ -301,20 +337,21 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         const int64_t blk = block_sum(my_found, sm);
         if (topdown) c.found_td += blk;
         else c.found_bu += blk;
-        const int fslot = (L & 1) ? kFound1 : kFound0;
-        const int mslot = (L & 1) ? kMin1 : kMin0;
+        int64_t key = 0;  // BW: INT32_MAX - (smallest id reached); 0 = none
         if (BW) {
             const int32_t m = wave_min(gmin);
             if (c.lane == 0) sm.tmp[c.wave] = m;
             __syncthreads();
+            int32_t bm = c.lane < kW ? sm.tmp[c.lane] : INT32_MAX;
+#pragma unroll
+            for (int d = kW / 2; d > 0; d >>= 1) bm = min(bm, __shfl_xor(bm, d, kWave));
+            bm = __shfl(bm, 0, kWave);
+            key = (int64_t)INT32_MAX - bm;
         }
-        if (c.tid == 0) {
-            st_agent(&a.part[(int64_t)fslot * c.G + c.b], (int64_t)(blk));
-            if (BW) st_agent(&a.part[(int64_t)mslot * c.G + c.b], (int64_t)(min(min(sm.tmp[0], sm.tmp[1]), min(sm.tmp[2], sm.tmp[3]))));
-        }
-        if (!gsync(gs, c.G, sm, a.dbg)) return -1;
-        const int64_t found = sum_slot(a.part, fslot, c.G, sm);
-        if (BW) *gmin_out = (int32_t)min_slot(a.part, mslot, c.G, sm);
+        Red r;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{blk, 0, key}, &r)) return -1;
+        const int64_t found = r.s0;
+        if (BW) *gmin_out = (int32_t)((int64_t)INT32_MAX - r.key);
         visited += found;
         ++L;
         if (found == 0) break;
@@ -350,14 +387,14 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
 
 // One trim-1 sweep (trim_1_from_fq_gfq, src/trim_1_gfq.h:365-458) over
 // fin[0, n): act is updated in place, survivors are pushed to fout.
-// Returns the number trimmed (block partial written to `slot`).
-__device__ __forceinline__ void trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount, int slot) {
+// Returns the number this block trimmed.
+__device__ __forceinline__ int64_t trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount) {
     const GiantArgs& a = c.a;
     WaveQueue wq{c.sm.wbuf[c.wave], 0};
     const uint64_t* act = a.act;
     auto live = [act](int32_t w, int32_t self) { return w != self && bit_of(act, w); };
     int trimmed = 0, scanned = 0;
-    for (int64_t ib = (int64_t)c.b * kBlock + c.wave * kWave; ib < n; ib += (int64_t)c.G * kBlock) {
+    for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < n; ib += (int64_t)c.G * kGBlock) {
         const int64_t i = ib + c.lane;
         int32_t v = 0;
         bool alive = false;
@@ -377,8 +414,7 @@ __device__ __forceinline__ void trim_sweep(Ctx& c, const int32_t* fin, int64_t n
         wq_push<true>(wq, alive && !trim, v, fout, fcount);
     }
     wq_flush<true>(wq, fout, fcount);
-    const int64_t t = block_sum(trimmed, c.sm);
-    if (c.tid == 0) st_agent(&a.part[(int64_t)slot * c.G + c.b], (int64_t)(t));
+    return block_sum(trimmed, c.sm);
 }
 
 __device__ __forceinline__ int32_t uf_root_g(int32_t* parent, int32_t v) {
@@ -398,18 +434,18 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
     auto buf = [&a](int k) { return k ? a.fq : a.fq_alt; };
     int64_t prev = fq0, trim1 = 0, p2 = 0, p3 = 0;
     int it = 0;
-    const int64_t nthreads = (int64_t)c.G * kBlock;
-    const int64_t gt = (int64_t)c.b * kBlock + c.tid;
+    const int64_t nthreads = (int64_t)c.G * kGBlock;
+    const int64_t gt = (int64_t)c.b * kGBlock + c.tid;
     // Sweep s pushes into fqn[s % 3]; all blocks read it after the barrier.
     // It clears fqn[(s + 1) % 3] for the next sweep: that counter was last read
     // after sweep s - 2, i.e. before the barrier that ended sweep s - 1.
     auto sweep = [&]() -> bool {
         if (c.b == 0 && c.tid == 0)
             __hip_atomic_store(&gs->fqn[(s + 1) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int slot = (s & 1) ? kTrim1 : kTrim0;
-        trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v, slot);
-        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
-        trim1 += sum_slot(a.part, slot, c.G, c.sm);
+        const int64_t t = trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v);
+        Red r;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{t, 0, 0}, &r)) return false;
+        trim1 += r.s0;
         n = load_agent(&gs->fqn[s % 3].v);
         cur ^= 1;
         ++s;
@@ -442,13 +478,10 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
             }
         }
         const int64_t b2 = block_sum(c2, c.sm), b3 = block_sum(c3, c.sm);
-        if (c.tid == 0) {
-            st_agent(&a.part[(int64_t)kPat2 * c.G + c.b], (int64_t)(b2));
-            st_agent(&a.part[(int64_t)kPat3 * c.G + c.b], (int64_t)(b3));
-        }
-        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
-        p2 = sum_slot(a.part, kPat2, c.G, c.sm);
-        p3 = sum_slot(a.part, kPat3, c.G, c.sm);
+        Red r;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{b2, b3, 0}, &r)) return false;
+        p2 = r.s0;
+        p3 = r.s1;
         // P11: one more trim-1 sweep
         if (!sweep()) return false;
         if (n > 0) {
@@ -501,7 +534,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
 
 }  // namespace
 
-__global__ __launch_bounds__(kBlock) void k_giant(const GiantArgs a) {
+__global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     __shared__ Smem sm;
     if (threadIdx.x == 0) {
         a.dbg[blockIdx.x] = 0;
@@ -569,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_giant(const GiantArgs a) {
     // every block reduces the pivot candidates and the survivor count
     bestp = 0;
     besti = INT32_MAX;
-    for (int i = c.tid; i < c.G; i += kBlock) {
+    for (int i = c.tid; i < c.G; i += kGBlock) {
         const int64_t p = load_agent(&a.part[(int64_t)kPivP * c.G + i]);
         const int32_t id = (int32_t)load_agent(&a.part[(int64_t)kPivI * c.G + i]);
         if (pivot_better(p, id, bestp, besti)) {
@@ -646,7 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_giant(const GiantArgs a) {
     int64_t off, tot;
     {
         int64_t x0 = 0, xt = 0;
-        for (int i = c.tid; i < c.G; i += kBlock) {
+        for (int i = c.tid; i < c.G; i += kGBlock) {
             const int64_t x = load_agent(&a.part[(int64_t)kPost * c.G + i]);
             xt += x;
             if (i < c.b) x0 += x;
@@ -654,11 +687,11 @@ __global__ __launch_bounds__(kBlock) void k_giant(const GiantArgs a) {
         off = block_sum(x0, sm);
         tot = block_sum(xt, sm);
     }
-    for (int64_t base = wb; base < we; base += kBlock) {
+    for (int64_t base = wb; base < we; base += kGBlock) {
         const int64_t w = base + c.tid;
         const uint64_t m = w < we ? load_agent(&a.act[w]) : 0ull;
         int excl, total;
-        block_excl_scan(__popcll(m), excl, total, sm.tmp);
+        block_excl_scan<kW>(__popcll(m), excl, total, sm.tmp);
         uint64_t mm = m;
         int64_t o = off + excl;
         while (mm) {
@@ -709,7 +742,7 @@ __global__ void k_first(const int32_t* __restrict__ beg, const int32_t* __restri
 
 int giant_max_blocks_per_cu() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kBlock, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kGBlock, 0) != hipSuccess) return 0;
     return nb;
 }
 
@@ -720,7 +753,7 @@ hipError_t launch_giant(const GiantArgs& a, int grid, hipStream_t s) {
     if (e != hipSuccess) return e;
     GiantArgs copy = a;
     void* args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kBlock), args, 0, s);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args, 0, s);
 }
 
 hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s) {

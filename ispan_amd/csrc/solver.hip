@@ -770,6 +770,16 @@ int ispan_ctx_set_stream(ispan_ctx* ctx, void* hip_stream) {
     return ISPAN_OK;
 }
 
+int ispan_ctx_synchronize(ispan_ctx* ctx) {
+    if (!ctx) return ISPAN_ERR_ARG;
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+        return ISPAN_ERR_HIP;
+    }
+    return ISPAN_OK;
+}
+
 const char* ispan_ctx_last_error(const ispan_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int ispan_graph_upload(ispan_ctx* ctx, int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj,

@@ -100,12 +100,7 @@ def test_graph_load_roundtrip_and_errors(tmp_path):
 
 
 def test_context_without_gpu_fails_cleanly():
-    try:
-        import torch
-        has_gpu = torch.cuda.is_available()
-    except Exception:
-        has_gpu = False
-    if has_gpu:
+    if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU is present")
     with pytest.raises(I.IspanError) as e:
         I.Solver(0)

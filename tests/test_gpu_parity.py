@@ -12,6 +12,7 @@ import pytest
 import ispan_amd as I
 import oracle as O
 from tests import graphs as G
+from tests import hip_rt as H
 
 pytestmark = pytest.mark.gpu
 
@@ -157,26 +158,43 @@ def test_one_shot_entry_point():
 
 
 def test_labels_device_copy(solver):
-    import torch
     g = O.rmat(12, 8, 6)
     solver.upload(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj))
-    out = torch.full((g.V,), -7, dtype=torch.int32, device="cuda")
-    solver.solve(I.default_params(), d_labels=out.data_ptr())
-    torch.cuda.synchronize()
-    assert np.array_equal(out.cpu().numpy(), O.tarjan(g))
+    out = H.DeviceArray(np.full(g.V, -7, np.int32))
+    try:
+        solver.solve(I.default_params(), d_labels=out.ptr)
+        solver.synchronize()
+        assert np.array_equal(out.download(), O.tarjan(g))
+    finally:
+        out.free()
     assert solver.labels_device_ptr() != 0
 
 
 def test_external_stream(solver):
-    import torch
     g = O.rmat(13, 8, 8)
-    st = torch.cuda.Stream()
-    solver.set_stream(st.cuda_stream)
+    st = H.Stream()
+    solver.set_stream(st.handle)
     try:
         lab, _ = solve(solver, g)
+        st.synchronize()
     finally:
         solver.set_stream(None)
+        st.destroy()
     assert np.array_equal(lab, O.tarjan(g))
+
+
+def test_bind_caller_device_csr(solver):
+    """Solve on a CSR that lives in caller-allocated device memory."""
+    g = O.rmat(14, 8, 9)
+    bufs = [H.DeviceArray(np.asarray(a, np.int32)) for a in (g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)]
+    try:
+        solver.bind_device(g.V, g.E, *[b.ptr for b in bufs])
+        solver.solve(I.default_params())
+        assert np.array_equal(solver.labels(), O.tarjan(g))
+    finally:
+        for b in bufs:
+            b.free()
+    solver.upload(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj))  # drop the freed pointers
 
 
 def test_stats_consistency(solver):
@@ -219,13 +237,12 @@ def test_hashed_graphs_on_device(solver, key, d):
 
 
 def test_build_csr_device_matches_oracle(solver):
-    import torch
     rng = np.random.default_rng(12)
     V, m = 5000, 40000
     e = rng.integers(0, V, size=(m, 2)).astype(np.int32)
-    src = torch.from_numpy(e[:, 0].copy()).cuda()
-    dst = torch.from_numpy(e[:, 1].copy()).cuda()
-    dg = solver.build_csr(V, m, src.data_ptr(), dst.data_ptr())
+    src = H.DeviceArray(e[:, 0].copy())
+    dst = H.DeviceArray(e[:, 1].copy())
+    dg = solver.build_csr(V, m, src.ptr, dst.ptr)
     try:
         hg = dg.download()
         og = O.build_csr(V, e[:, 0].copy(), e[:, 1].copy())
@@ -233,10 +250,15 @@ def test_build_csr_device_matches_oracle(solver):
             assert np.array_equal(a, b)
     finally:
         dg.free()
-    dst[7] = V  # out of range
-    with pytest.raises(I.IspanError) as ex:
-        solver.build_csr(V, m, src.data_ptr(), dst.data_ptr())
-    assert ex.value.code == -2
+    e[7, 1] = V  # out of range
+    dst.upload(e[:, 1].copy())
+    try:
+        with pytest.raises(I.IspanError) as ex:
+            solver.build_csr(V, m, src.ptr, dst.ptr)
+        assert ex.value.code == -2
+    finally:
+        src.free()
+        dst.free()
 
 
 # ----------------------------------------------------------- drop-in CLI
