@@ -32,6 +32,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from ispan_amd import hiprt  # noqa: E402
+
 METRIC = "edges/s + HBM GB/s per SCC solve; batched graphs/s, p50/p99 latency vs 1/launch"
 HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
@@ -58,11 +60,21 @@ def build_graph(S, workload: str, seed: int):
 
 
 def algorithmic_bytes(cls: int, kp, V: int) -> float:
-    """Per-class algorithmic bytes: 4 per adjacency entry read, 12 per vertex
-    reached; trim-1-first reads both offset arrays and writes label + queue."""
+    """Multi-launch engine, per kernel class: 4 B per adjacency entry read,
+    12 B per vertex reached; trim-1-first reads both offset arrays and writes
+    label + queue (12 B per vertex) plus 4 B per queued vertex."""
     if cls == 3:
         return 12.0 * V * kp.launches[3] + 4.0 * kp.found[3]
     return 4.0 * kp.edges[cls] + 12.0 * kp.found[cls]
+
+
+def giant_kernel_bytes(st, V: int) -> float:
+    """Cooperative engine: the one k_giant launch per solve runs trim-1-first
+    (12 B per vertex + 4 B per queued vertex) and both traversals (4 B per
+    adjacency entry of a reached vertex in the traversal direction + 12 B per
+    reached vertex).  The trims and WCC after the giant are not counted, so
+    the achieved rate is a lower bound."""
+    return 12.0 * V + 4.0 * st.fq0 + 4.0 * (st.fw_edges + st.bw_edges) + 12.0 * (st.fw_size + st.giant_size)
 
 
 def cpu_leg(dg, V: int, E: int, gpu_labels: np.ndarray, threads: int, runs: int, timeout: float) -> dict:
@@ -117,7 +129,11 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--timing", type=int, default=2, help="library timing level inside the timed region")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration, 5)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs the SCC pipeline on a single MI355X "
               "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
@@ -187,22 +203,33 @@ def main(argv=None) -> int:
                      "fw_size": st_phase.fw_size, "giant": st_phase.giant_size, "fw_edges": st_phase.fw_edges,
                      "bw_edges": st_phase.bw_edges, "host_syncs": st_phase.host_syncs, "n_scc": st_phase.n_scc}
     if a.timing >= 2 and sum(launches):
-        dom = max(range(4), key=lambda c: prof[c])
         kp_tot = I.KernelProfile()
         for c in range(4):
             kp_tot.launches[c], kp_tot.ms[c], kp_tot.edges[c], kp_tot.found[c] = launches[c], prof[c], edges[c], \
                 found[c]
-        per_launch_bytes = algorithmic_bytes(dom, kp_tot, V) / launches[dom]
+        if st_phase.engine_used == 0:
+            dom, name = 3, "k_giant"
+            per_launch_bytes = giant_kernel_bytes(st_phase, V)
+            unit_note = ("12*V + 4*fq0 + 4*(E_FW + E_BW) + 12*(|FW| + |giant|) per launch "
+                         "(trim-1-first + both traversals; later trims/WCC not counted)")
+        else:
+            dom = max(range(4), key=lambda c: prof[c])
+            name = I.KernelProfile.CLASSES[dom]
+            per_launch_bytes = algorithmic_bytes(dom, kp_tot, V) / launches[dom]
+            unit_note = "4 B per adjacency entry read + 12 B per vertex reached, per launch"
         avg_ms = prof[dom] / launches[dom]
         achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-        result["roofline"] = {"bound": "hbm", "kernel": I.KernelProfile.CLASSES[dom], "achieved": achieved,
+        result["roofline"] = {"bound": "hbm", "kernel": name, "achieved": achieved,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                              "traffic": None, "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms,
-                              "launches": launches[dom]}
-        result["kernels"] = {I.KernelProfile.CLASSES[c]: {"launches": launches[c], "ms_total": prof[c],
-                                                          "gbps": algorithmic_bytes(c, kp_tot, V) / (prof[c] * 1e-3)
-                                                          / 1e9 if prof[c] > 0 else None}
-                             for c in range(4)}
+                              "traffic": None, "bytes_per_launch": per_launch_bytes, "bytes_model": unit_note,
+                              "avg_launch_ms": avg_ms, "launches": launches[dom],
+                              "timing": "HIP events on the solver stream around each launch, inside the timed region"}
+        result["engine"] = int(st_phase.engine_used)
+        if st_phase.engine_used != 0:
+            result["kernels"] = {I.KernelProfile.CLASSES[c]: {"launches": launches[c], "ms_total": prof[c],
+                                                              "gbps": algorithmic_bytes(c, kp_tot, V) / (prof[c] * 1e-3)
+                                                              / 1e9 if prof[c] > 0 else None}
+                                 for c in range(4)}
         pmc = os.path.join(ROOT, "profiles", f"pmc_{result['roofline']['kernel']}_{a.workload}.json")
         if os.path.exists(pmc):
             with open(pmc) as f:
@@ -227,6 +254,24 @@ def main(argv=None) -> int:
     dg.free()
     S.close()
     print(json.dumps(result), flush=True)
+    hiprt.release_device()
+    return 0
+
+
+def copy_calibration(nbytes: int, reps: int) -> int:
+    """Device-to-device copies of a known size: under rocprofv3 --pmc, the
+    FETCH_SIZE / WRITE_SIZE of the runtime's copy kernel per dispatch give
+    the factor that turns those counters into bytes."""
+    a, b = hiprt.malloc(nbytes), hiprt.malloc(nbytes)
+    try:
+        for _ in range(reps):
+            hiprt.device_copy(b, a, nbytes)
+        hiprt.synchronize()
+    finally:
+        hiprt.free(a)
+        hiprt.free(b)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+    hiprt.release_device()
     return 0
 
 

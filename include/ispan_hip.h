@@ -87,7 +87,8 @@ typedef struct ispan_stats {
     int64_t bw_levels;    /* BW traversal levels                             */
     int64_t n_scc;        /* number of SCCs (computed on request)            */
     int64_t host_syncs;   /* host<->device round trips in this solve         */
-    int64_t reserved[9];
+    int64_t engine_used;  /* giant-phase engine that ran (see ispan_params)  */
+    int64_t reserved[8];
 } ispan_stats;
 
 typedef struct ispan_ctx ispan_ctx;

@@ -85,7 +85,11 @@ struct GiantState {
     RLine gacc[32];     // barrier reductions: per arrival group
     RLine tacc;         // barrier reductions: over groups
     RLine res[2];       // barrier reductions: result of generation g in res[g & 1]
+    // per level of each direction (first kLevelTrace levels): end time,
+    // vertices found, adjacency entries read, 0 top-down / 1 bottom-up
+    int64_t lv_ts[2][64], lv_found[2][64], lv_edges[2][64], lv_mode[2][64];
 };
+constexpr int kLevelTrace = 64;
 
 struct GiantArgs {
     const int32_t* fwb;

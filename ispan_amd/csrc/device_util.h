@@ -118,6 +118,41 @@ __device__ __forceinline__ bool try_claim(int32_t w, const uint64_t* __restrict_
     return !(old & bit);
 }
 
+// Wave-wide claim of targets w (lanes with want): only allowed, not yet
+// visited vertices are claimed, each by exactly one lane.  Consecutive lanes
+// whose targets share a bitmap word form a run; an OR-scan over the run gives
+// one atomic per run instead of one per lane, so sorted lists that pack many
+// targets into few words (a hub's list, a grid row) do not serialise on one
+// address.  The exclusive part of the scan keeps a target repeated inside a
+// run from being claimed twice.  Must be called by all 64 lanes.
+__device__ __forceinline__ bool wave_try_claim(bool want, int32_t w, const uint64_t* __restrict__ allow,
+                                               uint64_t* vis) {
+    const int lane = lane_id();
+    const int32_t wi = w >> 6;
+    const uint64_t bit = 1ull << (w & 63);
+    bool cand = false;
+    if (want) cand = (allow[wi] & bit) && !(vis[wi] & bit);
+    const int32_t key = cand ? wi : -1 - lane;  // lanes without a candidate are runs of their own
+    const int32_t pkey = __shfl_up(key, 1, kWave);
+    const uint64_t heads = __ballot(lane == 0 || pkey != key);
+    const uint64_t le = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes <= this one
+    const int start = 63 - __clzll(heads & le);
+    const uint64_t after = heads & ~le;
+    const int end = after ? __ffsll((unsigned long long)after) - 2 : kWave - 1;
+    uint64_t incl = cand ? bit : 0ull;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t y = __shfl_up(incl, d, kWave);
+        if (lane - d >= start) incl |= y;
+    }
+    uint64_t excl = __shfl_up(incl, 1, kWave);
+    if (lane == start) excl = 0;
+    unsigned long long old = 0;
+    if (lane == end && incl) old = atomicOr(reinterpret_cast<unsigned long long*>(vis + wi), (unsigned long long)incl);
+    old = __shfl(old, end, kWave);
+    return cand && !((old | excl) & bit);
+}
+
 // Does any edge in adj[p, end) satisfy pred(w, self)?  Each lane first walks
 // its own list for up to kLaneEdges edges; the lists still open after that
 // are finished by the whole wave, 64 edges per step, one list at a time.

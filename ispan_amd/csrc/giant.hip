@@ -37,6 +37,9 @@ struct Smem {
     int32_t bi[kW];
     int ok;
     int32_t wbuf[kW][kWaveBuf];
+    int32_t hv[kW][kWave];  // per-wave staged hub vertices: id, first edge, degree
+    int32_t hs[kW][kWave];
+    int32_t hd[kW][kWave];
 };
 
 struct Ctx {
@@ -162,24 +165,64 @@ __device__ __forceinline__ int64_t sum_slot(const int64_t* part, int slot, int G
     return block_sum(x, sm);
 }
 
+// Per-wave staging of hub vertices in LDS; a flush turns them into
+// kGiantChunk-edge work items with one atomic for the whole batch.
+struct HubStage {
+    int32_t* v;
+    int32_t* s;
+    int32_t* d;
+    int n;  // wave-uniform fill
+};
+
+__device__ __forceinline__ void hub_flush(HubStage& hs, HubItem* hout, int32_t* hn_out) {
+    if (hs.n == 0) return;
+    wave_sync();
+    const int lane = lane_id();
+    int32_t v = 0, s0 = 0, nch = 0;
+    if (lane < hs.n) {
+        v = hs.v[lane];
+        s0 = hs.s[lane];
+        nch = (hs.d[lane] + kGiantChunk - 1) / kGiantChunk;
+    }
+    const int incl = wave_incl_scan(nch);
+    int base = 0;
+    if (lane == kWave - 1) base = atomicAdd(hn_out, incl);
+    base = __shfl(base, kWave - 1, kWave) + incl - nch;
+    for (int k = 0; k < nch; ++k) {
+        st_agent(&hout[base + k].v, v);
+        st_agent(&hout[base + k].start, s0 + k * kGiantChunk);
+    }
+    hs.n = 0;
+    wave_sync();
+}
+
+__device__ __forceinline__ void hub_push(HubStage& hs, bool pred, int32_t v, int32_t s0, int32_t d, HubItem* hout,
+                                         int32_t* hn_out) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int c = __popcll(m);
+    if (hs.n + c > kWave) hub_flush(hs, hout, hn_out);
+    if (pred) {
+        const int i = hs.n + __popcll(m & lanemask_lt());
+        hs.v[i] = v;
+        hs.s[i] = s0;
+        hs.d[i] = d;
+    }
+    hs.n += c;
+}
+
 // Pushes a newly claimed vertex for expansion at the next level: vertices of
 // degree >= kGiantHubDeg become kGiantChunk-edge work items, others go to
 // the queue (degree-0 vertices have nothing to expand).  All lanes call it.
 __device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* __restrict__ beg, WaveQueue& wq,
-                                          int32_t* qout, int32_t* qn_out, HubItem* hout, int32_t* hn_out) {
+                                          int32_t* qout, int32_t* qn_out, HubStage& hs, HubItem* hout,
+                                          int32_t* hn_out) {
     int32_t d = 0, s0 = 0;
     if (claim) {
         s0 = beg[w];
         d = beg[w + 1] - s0;
-        if (d >= kGiantHubDeg) {
-            const int nch = (d + kGiantChunk - 1) / kGiantChunk;
-            const int h = atomicAdd(hn_out, nch);
-            for (int c = 0; c < nch; ++c) {
-                st_agent(&hout[h + c].v, w);
-                st_agent(&hout[h + c].start, s0 + c * kGiantChunk);
-            }
-        }
     }
+    hub_push(hs, claim && d >= kGiantHubDeg, w, s0, d, hout, hn_out);
     wq_push<true>(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
 }
 
@@ -200,6 +243,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
     GiantState* gs = a.gs;
     const int dir = BW ? 1 : 0;
     WaveQueue wq{sm.wbuf[c.wave], 0};
+    HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
 
     int64_t visited = 0;
     int32_t gmin = BW ? root : INT32_MAX;
@@ -221,6 +265,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             __hip_atomic_store(&gs->hn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         const bool topdown = mode != 1;
+        const int64_t edges0 = c.edges_td + c.edges_bu;  // thread 0's are the block's
         int64_t my_found = 0;  // per thread
         if (topdown) {
             // ---- top-down: tiles of the queue, then hub work items; level 0
@@ -257,7 +302,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     __syncthreads();
                     for (int eb = 0; eb < total; eb += kGBlock) {
                         const int e = eb + c.tid;
-                        bool claim = false;
+                        bool want = false;
                         int32_t w = 0;
                         if (e < total) {
                             int lo = 0, hi = kGBlock - 1;  // last j with off[j] <= e
@@ -267,11 +312,12 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                                 else hi = mid - 1;
                             }
                             w = adj[sm.start[lo] + (e - sm.off[lo])];
-                            claim = w != root && try_claim(w, allow, vis);
+                            want = w != root;
                         }
+                        const bool claim = wave_try_claim(want, w, allow, vis);
                         my_found += claim;
                         if (BW && claim) gmin = min(gmin, w);
-                        push_next(claim, w, beg, wq, qout, qn_out, hout, hn_out);
+                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out);
                     }
                     __syncthreads();
                 } else {
@@ -287,49 +333,69 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     if (c.tid == 0) c.edges_td += e1 - s0;
                     for (int32_t eb = s0; eb < e1; eb += kGBlock) {
                         const int32_t e = eb + c.tid;
-                        bool claim = false;
+                        bool want = false;
                         int32_t w = 0;
                         if (e < e1) {
                             w = adj[e];
-                            claim = w != root && try_claim(w, allow, vis);
+                            want = w != root;
                         }
+                        const bool claim = wave_try_claim(want, w, allow, vis);
                         my_found += claim;
                         if (BW && claim) gmin = min(gmin, w);
-                        push_next(claim, w, beg, wq, qout, qn_out, hout, hn_out);
+                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out);
                     }
                 }
             }
             wq_flush<true>(wq, qout, qn_out);
+            hub_flush(hs, hout, hn_out);
         } else {
-            // ---- bottom-up over the candidate bitmap, one wave per 64-vertex
-            // word; only that wave writes the word here.  Its visited bits
-            // before the sweep go to snap, so a later switch to top-down can
-            // gather this level's finds.
+            // ---- bottom-up over the candidate bitmap, one wave per group of
+            // kU consecutive 64-vertex words; only that wave writes those
+            // words in this level.  Each stage issues its kU loads (first
+            // in-neighbours, their visited bits) before any result is used.  Found vertices also go to the next queue, so
+            // a switch back to top-down needs no extra pass.
+            constexpr int kU = 4;
             int scanned = 0;
-            for (int64_t w = gwave; w < a.nwords; w += total_waves) {
-                const uint64_t vw = load_agent(&vis[w]);
-                const uint64_t cand = load_agent(&allow[w]) & ~vw;
-                if (c.lane == 0) st_agent(&a.snap[w], vw);
-                if (!cand) continue;
-                const int32_t v = (int32_t)(w * 64 + c.lane);
-                const bool is_c = (cand >> c.lane) & 1ull;
-                bool found = false;
-                int32_t p = 0, e = 0;
-                if (is_c) {
-                    const int32_t f = rfirst[v];
-                    found = f >= 0 && bit_of(vis, f);
-                    scanned += 1;
-                    if (!found) {
-                        p = rbeg[v] + 1;
-                        e = rbeg[v + 1];
-                    }
+            for (int64_t w0 = gwave * kU; w0 < a.nwords; w0 += total_waves * kU) {
+                // lane u < kU holds word w0 + u; the others read it by shuffle
+                uint64_t my_vw = 0, my_cand = 0;
+                if (c.lane < kU && w0 + c.lane < a.nwords) {
+                    my_vw = load_agent(&vis[w0 + c.lane]);
+                    my_cand = load_agent(&allow[w0 + c.lane]) & ~my_vw;
+                    st_agent(&a.snap[w0 + c.lane], my_vw);
                 }
-                found = wave_any_edge(is_c && !found, p, e, v, radj, is_vis, scanned) || found;
-                const uint64_t fm = __ballot(found);
-                if (fm && c.lane == 0)
-                    atomicOr(reinterpret_cast<unsigned long long*>(vis + w), (unsigned long long)fm);
-                my_found += found;
-                if (BW && found) gmin = min(gmin, v);
+                int32_t f[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    f[u] = -1;
+                    if ((__shfl(my_cand, u, kWave) >> c.lane) & 1ull) f[u] = rfirst[(w0 + u) * 64 + c.lane];
+                }
+                uint32_t hits = 0;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) hits |= (f[u] >= 0 && bit_of(vis, f[u])) ? 1u << u : 0u;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const uint64_t cm = __shfl(my_cand, u, kWave);
+                    if (!cm) continue;
+                    const int64_t w = w0 + u;
+                    const int32_t v = (int32_t)(w * 64 + c.lane);
+                    const bool is_c = (cm >> c.lane) & 1ull;
+                    const bool hit = (hits >> u) & 1u;
+                    int32_t p = 0, e = 0;
+                    if (is_c) {
+                        scanned += 1;
+                        if (!hit) {
+                            p = rbeg[v] + 1;
+                            e = rbeg[v + 1];
+                        }
+                    }
+                    const bool found = wave_any_edge(is_c && !hit, p, e, v, radj, is_vis, scanned) || (is_c && hit);
+                    const uint64_t fm = __ballot(found);
+                    const uint64_t vwu = __shfl(my_vw, u, kWave);
+                    if (fm && c.lane == 0) st_agent(&vis[w], vwu | fm);
+                    my_found += found;
+                    if (BW && found) gmin = min(gmin, v);
+                }
             }
             c.edges_bu += block_sum(scanned, sm);
         }
@@ -349,8 +415,14 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             key = (int64_t)INT32_MAX - bm;
         }
         Red r;
-        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{blk, 0, key}, &r)) return -1;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{blk, c.edges_td + c.edges_bu - edges0, key}, &r)) return -1;
         const int64_t found = r.s0;
+        if (c.b == 0 && c.tid == 0 && L < kLevelTrace) {
+            gs->lv_ts[dir][L] = wall_clock64();
+            gs->lv_found[dir][L] = found;
+            gs->lv_edges[dir][L] = r.s1;
+            gs->lv_mode[dir][L] = topdown ? 0 : 1;
+        }
         if (BW) *gmin_out = (int32_t)((int64_t)INT32_MAX - r.key);
         visited += found;
         ++L;
@@ -364,17 +436,28 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             const double er = (double)(set_size - visited) * a.avg_degree;
             if (er / a.alpha < ef) mode = 1;
         } else if (mode == 1 && ((set_size * 1.0 / a.beta) > found || L - 1 > a.gamma)) {
-            // gather this level's finds (vis & ~snap) as the next top-down input
+            // gather this level's finds (vis & ~snap) as the next top-down
+            // input: each wave loads 64 words at once, then expands the
+            // non-empty ones
             int32_t* q_next = (L & 1) ? a.q1 : a.q0;
             HubItem* h_next = (L & 1) ? a.h1 : a.h0;
             int32_t* qn_next = &gs->qn[dir][L % 3].v;
             int32_t* hn_next = &gs->hn[dir][L % 3].v;
-            for (int64_t w = gwave; w < a.nwords; w += total_waves) {
-                const uint64_t nw = load_agent(&vis[w]) & ~load_agent(&a.snap[w]);
-                const int32_t v = (int32_t)(w * 64 + c.lane);
-                push_next((nw >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, h_next, hn_next);
+            for (int64_t wb = gwave * kWave; wb < a.nwords; wb += total_waves * kWave) {
+                const int64_t w = wb + c.lane;
+                uint64_t nw = 0;
+                if (w < a.nwords) nw = load_agent(&vis[w]) & ~load_agent(&a.snap[w]);
+                uint64_t busy = __ballot(nw != 0);
+                while (busy) {
+                    const int l = __ffsll((unsigned long long)busy) - 1;
+                    busy &= busy - 1;
+                    const uint64_t m = __shfl(nw, l, kWave);
+                    const int32_t v = (int32_t)((wb + l) * 64 + c.lane);
+                    push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next);
+                }
             }
             wq_flush<true>(wq, q_next, qn_next);
+            hub_flush(hs, h_next, hn_next);
             mode = 2;
             if (!gsync(gs, c.G, sm, a.dbg)) return -1;
         }

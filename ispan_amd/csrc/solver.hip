@@ -66,6 +66,7 @@ struct ispan_ctx {
     Ctl* h_ctl = nullptr;  // pinned
     // cooperative giant-phase kernel
     int coop_grid = 0;
+    bool level_trace = false;  // ISPAN_LEVEL_TRACE=1: per-level lines of the giant phase on stderr
     double clock_khz = 0.0;
     uint64_t* snap = nullptr;
     HubItem* gh[2] = {nullptr, nullptr};
@@ -350,12 +351,12 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     int32_t root = -1;
     bool rest_done = false;
     double giant_phase_ms[4] = {0, 0, 0, 0};  // engine 0: trim-1-first+pivot, FW, BW+relabel, trims+WCC
-    if (P->engine == 0) {
+    // Engine 0 needs a cooperative launch (all blocks resident at once).  The
+    // device grants it to one process at a time; when it is refused (or the
+    // device has none) this solve runs engine 1, and stats.engine_used says so.
+    bool coop = P->engine == 0 && ctx->coop_grid > 0;
+    if (coop) {
         // P1-P6 + relabel in one cooperative kernel
-        if (ctx->coop_grid <= 0) {
-            ctx->err = "cooperative launch is not available on this device";
-            return ISPAN_ERR_HIP;
-        }
         GiantArgs ga{};
         ga.fwb = ctx->fwb;
         ga.fwa = ctx->fwa;
@@ -397,60 +398,82 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             ctx->recs.push_back({3, ga0, ga1});
             CK(hipEventRecord(ga0, s));
         }
-        CK(launch_giant(ga, ctx->coop_grid, s));
-        if (ga1) CK(hipEventRecord(ga1, s));
-        rec(ctx, P, 1);
-        rec(ctx, P, 2);
-        rec(ctx, P, 3);
-        rec(ctx, P, 4);
-        CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, sizeof(GiantState), hipMemcpyDeviceToHost, s));
-        if ((rc = readback(ctx, &syncs))) return rc;
-        const GiantState& gs = *ctx->h_gstate;
-        if (gs.err.v) {
-            if (gs.err.v & 1) {  // where did the blocks stop?
-                std::vector<int32_t> d((size_t)2 * ctx->coop_grid);
-                if (hipMemcpy(d.data(), ctx->gdbg, d.size() * 4, hipMemcpyDeviceToHost) == hipSuccess) {
-                    int amin = INT32_MAX, amax = 0, pmin = INT32_MAX, pmax = 0, zero = 0;
-                    for (int b = 0; b < ctx->coop_grid; ++b) {
-                        amin = std::min(amin, d[b]);
-                        amax = std::max(amax, d[b]);
-                        pmin = std::min(pmin, d[ctx->coop_grid + b]);
-                        pmax = std::max(pmax, d[ctx->coop_grid + b]);
-                        zero += d[b] == 0;
+        const hipError_t le = launch_giant(ga, ctx->coop_grid, s);
+        if (le == hipErrorCooperativeLaunchTooLarge || le == hipErrorIllegalState || le == hipErrorNotSupported ||
+            le == hipErrorLaunchOutOfResources) {
+            (void)hipGetLastError();
+            if (ga0) ctx->recs.pop_back();
+            coop = false;
+        } else {
+            CK(le);
+            if (ga1) CK(hipEventRecord(ga1, s));
+            rec(ctx, P, 1);
+            rec(ctx, P, 2);
+            rec(ctx, P, 3);
+            rec(ctx, P, 4);
+            CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, sizeof(GiantState), hipMemcpyDeviceToHost, s));
+            if ((rc = readback(ctx, &syncs))) return rc;
+            const GiantState& gs = *ctx->h_gstate;
+            if (gs.err.v) {
+                if (gs.err.v & 1) {  // where did the blocks stop?
+                    std::vector<int32_t> d((size_t)2 * ctx->coop_grid);
+                    if (hipMemcpy(d.data(), ctx->gdbg, d.size() * 4, hipMemcpyDeviceToHost) == hipSuccess) {
+                        int amin = INT32_MAX, amax = 0, pmin = INT32_MAX, pmax = 0, zero = 0;
+                        for (int b = 0; b < ctx->coop_grid; ++b) {
+                            amin = std::min(amin, d[b]);
+                            amax = std::max(amax, d[b]);
+                            pmin = std::min(pmin, d[ctx->coop_grid + b]);
+                            pmax = std::max(pmax, d[ctx->coop_grid + b]);
+                            zero += d[b] == 0;
+                        }
+                        fprintf(stderr, "[ispan] barrier timeout: grid %d, arrived min %d max %d (blocks at 0: %d), "
+                                "passed min %d max %d, gen %d\n", ctx->coop_grid, amin, amax, zero, pmin, pmax, gs.gen.v);
                     }
-                    fprintf(stderr, "[ispan] barrier timeout: grid %d, arrived min %d max %d (blocks at 0: %d), "
-                            "passed min %d max %d, gen %d\n", ctx->coop_grid, amin, amax, zero, pmin, pmax, gs.gen.v);
+                }
+                ctx->err = gs.err.v & 1   ? "giant phase: grid barrier timed out"
+                           : gs.err.v & 2 ? "giant phase: level cap exceeded"
+                                          : "giant phase: queue count out of range";
+                return gs.err.v & 2 ? ISPAN_ERR_ITER : ISPAN_ERR_HIP;
+            }
+            fq0 = gs.fq0;
+            root = (int32_t)gs.root;
+            fw_count = gs.fw_count;
+            n = gs.fq_n;
+            local.fq0 = fq0;
+            local.pivot = root;
+            local.fw_size = fw_count;
+            local.giant_size = gs.giant;
+            local.fw_levels = gs.fw_levels;
+            local.bw_levels = gs.bw_levels;
+            if (ctx->clock_khz > 0) {
+                const double k = 1.0 / ctx->clock_khz;  // ms per tick
+                giant_phase_ms[0] = (gs.ts[1] - gs.ts[0]) * k;
+                giant_phase_ms[1] = root >= 0 ? (gs.ts[2] - gs.ts[1]) * k : 0.0;
+                giant_phase_ms[2] = (gs.ts[4] - (root >= 0 ? gs.ts[2] : gs.ts[1])) * k;
+                giant_phase_ms[3] = gs.ts[5] > gs.ts[4] ? (gs.ts[5] - gs.ts[4]) * k : 0.0;
+            }
+            if (ctx->level_trace && ctx->clock_khz > 0) {
+                const double k = 1.0 / ctx->clock_khz;
+                for (int d = 0; d < 2; ++d) {
+                    int64_t t = d == 0 ? gs.ts[1] : gs.ts[2];
+                    const int64_t nl = std::min<int64_t>(d == 0 ? gs.fw_levels : gs.bw_levels, kLevelTrace);
+                    for (int64_t l = 0; l < nl; ++l) {
+                        fprintf(stderr, "[ispan] %s level %2lld %s found %10lld edges %11lld  %8.3f ms\n", d ? "BW" : "FW",
+                                (long long)l, gs.lv_mode[d][l] ? "BU" : "TD", (long long)gs.lv_found[d][l],
+                                (long long)gs.lv_edges[d][l], (gs.lv_ts[d][l] - t) * k);
+                        t = gs.lv_ts[d][l];
+                    }
                 }
             }
-            ctx->err = gs.err.v & 1   ? "giant phase: grid barrier timed out"
-                       : gs.err.v & 2 ? "giant phase: level cap exceeded"
-                                      : "giant phase: queue count out of range";
-            return gs.err.v & 2 ? ISPAN_ERR_ITER : ISPAN_ERR_HIP;
+            rest_done = true;
+            ctx->prof.edges[0] = gs.work[0];
+            ctx->prof.edges[2] = gs.work[1];
+            ctx->prof.found[0] = gs.work[2];
+            ctx->prof.found[2] = gs.work[3];
+            ctx->prof.found[3] = fq0;
         }
-        fq0 = gs.fq0;
-        root = (int32_t)gs.root;
-        fw_count = gs.fw_count;
-        n = gs.fq_n;
-        local.fq0 = fq0;
-        local.pivot = root;
-        local.fw_size = fw_count;
-        local.giant_size = gs.giant;
-        local.fw_levels = gs.fw_levels;
-        local.bw_levels = gs.bw_levels;
-        if (ctx->clock_khz > 0) {
-            const double k = 1.0 / ctx->clock_khz;  // ms per tick
-            giant_phase_ms[0] = (gs.ts[1] - gs.ts[0]) * k;
-            giant_phase_ms[1] = root >= 0 ? (gs.ts[2] - gs.ts[1]) * k : 0.0;
-            giant_phase_ms[2] = (gs.ts[4] - (root >= 0 ? gs.ts[2] : gs.ts[1])) * k;
-            giant_phase_ms[3] = gs.ts[5] > gs.ts[4] ? (gs.ts[5] - gs.ts[4]) * k : 0.0;
-        }
-        rest_done = true;
-        ctx->prof.edges[0] = gs.work[0];
-        ctx->prof.edges[2] = gs.work[1];
-        ctx->prof.found[0] = gs.work[2];
-        ctx->prof.found[2] = gs.work[3];
-        ctx->prof.found[3] = fq0;
-    } else {
+    }
+    if (!coop) {
         // P1-P3: trim-1-first, active queue, pivot
         hipEvent_t t1a = nullptr, t1b = nullptr;
         if (P->timing >= 2 && (t1a = pool_event(ctx)) && (t1b = pool_event(ctx))) {
@@ -592,6 +615,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     local.trim2 = pat2;
     local.trim3 = pat3;
     local.host_syncs = syncs;
+    local.engine_used = coop ? 0 : 1;
     local.ms[3] = wall_ms() - t0;
     if (P->timing) {
         // event map: 0 start, 1 trim-1-first+queue+pivot, 2 pivot readback,
@@ -689,7 +713,7 @@ void ispan_default_params(ispan_params* p) {
     p->thread_count = 1;
     p->validate = 1;
     p->async_batch = 8;
-    p->engine = 1;
+    p->engine = 0;
 }
 
 const char* ispan_strerror(int code) {
@@ -734,6 +758,7 @@ int ispan_ctx_create(ispan_ctx** out, int device) {
         int per_cu = coop ? std::min(giant_max_blocks_per_cu(), 8) : 0;
         if (const char* env = getenv("ISPAN_COOP_BLOCKS_PER_CU")) per_cu = std::min(per_cu, std::max(atoi(env), 1));
         ctx->coop_grid = per_cu * ctx->cu;
+        if (const char* env = getenv("ISPAN_LEVEL_TRACE")) ctx->level_trace = atoi(env) != 0;
     }
     if (hipMalloc(&ctx->gstate, sizeof(GiantState)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
     if (hipHostMalloc(&ctx->h_gstate, sizeof(GiantState), hipHostMallocDefault) != hipSuccess)

@@ -1,7 +1,8 @@
-"""Caller-side device memory and streams for the GPU tests, through the same
-HIP runtime library that libispan_hip.so is linked against (a second HIP
-runtime in the process, such as the one bundled with torch, cannot share the
-device with it)."""
+"""Caller-side device memory, streams and device release through the same HIP
+runtime library that libispan_hip.so is linked against (a second HIP runtime
+in the process, such as the one bundled with torch, cannot share the device
+with it).  Used by the GPU tests (caller-owned buffers and streams) and by
+bench.py (copy calibration, device release before exit)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -10,7 +11,7 @@ import numpy as np
 
 import ispan_amd as I
 
-_H2D, _D2H = 1, 2
+_H2D, _D2H, _D2D = 1, 2, 3
 _rt = None
 
 
@@ -22,7 +23,8 @@ def rt():
         for n, args in (("hipMalloc", [C.POINTER(C.c_void_p), C.c_size_t]), ("hipFree", [C.c_void_p]),
                         ("hipMemcpy", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
                         ("hipStreamCreate", [C.POINTER(C.c_void_p)]), ("hipStreamDestroy", [C.c_void_p]),
-                        ("hipStreamSynchronize", [C.c_void_p]), ("hipDeviceSynchronize", [])):
+                        ("hipStreamSynchronize", [C.c_void_p]), ("hipDeviceSynchronize", []),
+                        ("hipDeviceReset", [])):
             getattr(r, n).argtypes = args
             getattr(r, n).restype = C.c_int
         _rt = r
@@ -74,3 +76,30 @@ class Stream:
         if self.handle:
             _ok(rt().hipStreamDestroy(self.handle), "hipStreamDestroy")
             self.handle = None
+
+
+def device_copy(dst: int, src: int, nbytes: int):
+    """Device-to-device copy (the runtime's own copy kernel)."""
+    _ok(rt().hipMemcpy(dst, src, nbytes, _D2D), "hipMemcpy D2D")
+
+
+def malloc(nbytes: int) -> int:
+    p = C.c_void_p()
+    _ok(rt().hipMalloc(C.byref(p), max(nbytes, 1)), "hipMalloc")
+    return p.value
+
+
+def free(ptr: int):
+    _ok(rt().hipFree(ptr), "hipFree")
+
+
+def synchronize():
+    _ok(rt().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def release_device():
+    """Tear the device state of this process down now (every Solver must be
+    closed).  A process that ends right after it leaves nothing of the HIP
+    runtime for exit-time handlers; profilers that write their results at
+    exit need this after cooperative launches."""
+    _ok(rt().hipDeviceReset(), "hipDeviceReset")

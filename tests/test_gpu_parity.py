@@ -12,7 +12,7 @@ import pytest
 import ispan_amd as I
 import oracle as O
 from tests import graphs as G
-from tests import hip_rt as H
+from ispan_amd import hiprt as H
 
 pytestmark = pytest.mark.gpu
 
