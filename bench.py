@@ -287,4 +287,6 @@ def _cpu_model() -> str:
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    _rc = main()
+    if _rc:
+        sys.exit(_rc)

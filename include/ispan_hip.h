@@ -60,7 +60,12 @@ typedef struct ispan_params {
     int32_t engine;       /* giant phase: 0 one cooperative persistent kernel with
                              grid barriers (default), 1 one launch per level with
                              host-side switch decisions                              */
-    int32_t reserved[4];
+    int32_t switch_rule;  /* top-down -> bottom-up test, (set - visited) * avg_deg
+                             / alpha < frontier work: 1 (default) the frontier's
+                             work is its exact out-degree sum; 0 it is found *
+                             avg_deg, as src/fw_bw.h:578-581.  Engine 1 always
+                             uses 0.  Chooses the work, never the labels.       */
+    int32_t reserved[3];
 } ispan_params;
 
 /* Per-solve statistics.  ms[] uses the reference's avg_time slot map
@@ -153,6 +158,33 @@ int ispan_kernel_profile_get(const ispan_ctx* ctx, ispan_kernel_profile* out);
  * scc_detection + the label hook of get_scc_result).  labels: int32[V]. */
 int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
                         const int32_t* bw_adj, const ispan_params* params, int32_t* labels, ispan_stats* stats);
+
+/* ---- batched solves ----------------------------------------------------
+ * Many graphs in one pass of the pipeline over their disjoint union.  There
+ * is no counterpart in the reference, which solves one graph per process
+ * (src/scc_main.cpp:7-57); per graph the labels are those a single solve
+ * gives.  Layout (device memory, caller-owned): graph g has local ids
+ * 0..V_g-1; h_v_off / h_e_off (host, int64[n_graphs + 1], starting at 0) give
+ * V_g = v_off[g+1] - v_off[g] and E_g = e_off[g+1] - e_off[g]; its V_g + 1
+ * offsets (0 .. E_g) start at d_fw_beg[v_off[g] + g], its E_g adjacency
+ * entries at d_fw_adj[e_off[g]]; the bw pair likewise.
+ * Binding groups consecutive graphs whose union fits int32 offsets (at most
+ * max_group_edges edges per group; 0 = 2^31 - 1), builds each group's union
+ * CSR once and checks every graph (ISPAN_ERR_CSR).  The inputs may be freed
+ * after the call.  Binding replaces any bound graph or batch. */
+int ispan_batch_bind_device(ispan_ctx* ctx, int32_t n_graphs, const int64_t* h_v_off, const int64_t* h_e_off,
+                            const int32_t* d_fw_beg, const int32_t* d_fw_adj, const int32_t* d_bw_beg,
+                            const int32_t* d_bw_adj, int64_t max_group_edges, const ispan_params* params);
+/* Number of groups of the bound batch (0 if none). */
+int ispan_batch_groups(const ispan_ctx* ctx);
+/* Solve every graph of the bound batch: one cooperative launch per group,
+ * every graph's own pivot seeding the traversals.  Needs the cooperative
+ * engine.  stats: totals over the batch, ms[3] = wall time of the batch. */
+int ispan_batch_solve(ispan_ctx* ctx, const ispan_params* params, ispan_stats* stats);
+/* Labels of the last batch solve: int32[v_off[n_graphs]], graph g's at
+ * v_off[g], each the smallest local id of the vertex's SCC. */
+int ispan_batch_labels_download(ispan_ctx* ctx, int32_t* labels);
+const int32_t* ispan_batch_labels_device(const ispan_ctx* ctx);
 
 /* ---- reference file I/O and reporting ---------------------------------- */
 

@@ -36,7 +36,8 @@ EXPORTS = [
     "ispan_labels_device", "ispan_solve_stats", "ispan_scc_detection", "ispan_graph_load", "ispan_graph_free",
     "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
     "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
-    "ispan_convert_edge_list",
+    "ispan_convert_edge_list", "ispan_batch_bind_device", "ispan_batch_groups", "ispan_batch_solve",
+    "ispan_batch_labels_download", "ispan_batch_labels_device",
 ]
 
 
@@ -55,7 +56,8 @@ class Params(C.Structure):
     _fields_ = [("alpha", C.c_int32), ("beta", C.c_int32), ("gamma", C.c_int32), ("theta", C.c_double),
                 ("trim_times", C.c_int32), ("thread_count", C.c_int32), ("pivot_wrap32", C.c_int32),
                 ("timing", C.c_int32), ("validate", C.c_int32), ("wcc_mode", C.c_int32),
-                ("async_batch", C.c_int32), ("engine", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("async_batch", C.c_int32), ("engine", C.c_int32), ("switch_rule", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):
@@ -144,6 +146,12 @@ def lib():
                                              C.POINTER(C.c_int64)]),
         "ispan_device_free": (None, [_vp]),
         "ispan_kernel_profile_get": (C.c_int, [_vp, C.POINTER(KernelProfile)]),
+        "ispan_batch_bind_device": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int64,
+                                              C.POINTER(Params)]),
+        "ispan_batch_groups": (C.c_int, [_vp]),
+        "ispan_batch_solve": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Stats)]),
+        "ispan_batch_labels_download": (C.c_int, [_vp, _vp]),
+        "ispan_batch_labels_device": (_vp, [_vp]),
         "ispan_convert_edge_list": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int64)]),
     }
@@ -334,6 +342,31 @@ class Solver:
     def labels_device_ptr(self) -> int:
         return lib().ispan_labels_device(self._ctx) or 0
 
+    # ---- batched solves (many graphs, one pass over their disjoint union)
+    def batch_bind(self, batch: "DeviceBatch", max_group_edges: int = 0, params: Params | None = None):
+        v = np.ascontiguousarray(batch.v_off, np.int64)
+        e = np.ascontiguousarray(batch.e_off, np.int64)
+        _check(lib().ispan_batch_bind_device(self._ctx, len(v) - 1, _ptr(v), _ptr(e), batch.fw_beg, batch.fw_adj,
+                                             batch.bw_beg, batch.bw_adj, max_group_edges,
+                                             C.byref(params) if params else None), self._ctx)
+        self._batch_v_off = v
+        self.V = -1
+
+    def batch_groups(self) -> int:
+        return lib().ispan_batch_groups(self._ctx)
+
+    def batch_solve(self, params: Params | None = None) -> Stats:
+        st = Stats()
+        _check(lib().ispan_batch_solve(self._ctx, C.byref(params) if params else None, C.byref(st)), self._ctx)
+        return st
+
+    def batch_labels(self) -> list[np.ndarray]:
+        """Each graph's labels (smallest local id of the vertex's SCC)."""
+        v = self._batch_v_off
+        out = np.empty(max(int(v[-1]), 1), np.int32)
+        _check(lib().ispan_batch_labels_download(self._ctx, _ptr(out)), self._ctx)
+        return [out[v[g]:v[g + 1]] for g in range(len(v) - 1)]
+
     def gen_rmat(self, V: int, edge_factor: float = 16.0, seed: int = 1, a=0.57, b=0.19, c=0.19) -> DeviceGraph:
         ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]
         E = C.c_int64()
@@ -361,6 +394,71 @@ class Solver:
         _check(lib().ispan_build_csr_device(self._ctx, V, m, d_src, d_dst, *[C.byref(p) for p in ptrs],
                                             C.byref(E)), self._ctx)
         return DeviceGraph(V, E.value, *[C.cast(p, C.c_void_p).value or 0 for p in ptrs])
+
+
+class DeviceBatch:
+    """Graphs back to back in device memory, in the layout
+    ispan_batch_bind_device reads: graph g's V_g + 1 offsets at
+    fw_beg[v_off[g] + g], its adjacency at fw_adj[e_off[g]] (local ids)."""
+
+    def __init__(self, v_off, e_off, ptrs):
+        self.v_off = np.asarray(v_off, np.int64)
+        self.e_off = np.asarray(e_off, np.int64)
+        self.fw_beg, self.fw_adj, self.bw_beg, self.bw_adj = ptrs
+
+    @property
+    def n(self) -> int:
+        return len(self.v_off) - 1
+
+    @staticmethod
+    def _layout(sizes):
+        v_off = np.zeros(len(sizes) + 1, np.int64)
+        e_off = np.zeros(len(sizes) + 1, np.int64)
+        for i, (V, E) in enumerate(sizes):
+            v_off[i + 1] = v_off[i] + V
+            e_off[i + 1] = e_off[i] + E
+        return v_off, e_off
+
+    @classmethod
+    def from_host(cls, graphs: list[HostGraph]) -> "DeviceBatch":
+        from . import hiprt
+        sizes = [(len(g.fw_beg) - 1, len(g.fw_adj)) for g in graphs]
+        v_off, e_off = cls._layout(sizes)
+        n = len(graphs)
+        nb, na = int(v_off[-1] + n) * 4, int(e_off[-1]) * 4
+        ptrs = [hiprt.malloc(nb), hiprt.malloc(na), hiprt.malloc(nb), hiprt.malloc(na)]
+        for k, fld in enumerate(("fw_beg", "fw_adj", "bw_beg", "bw_adj")):
+            beg = fld.endswith("beg")
+            cat = np.concatenate([np.asarray(getattr(g, fld), np.int32) for g in graphs]) if n else \
+                np.empty(0, np.int32)
+            if cat.size:
+                hiprt.upload(ptrs[k], cat)
+            assert cat.nbytes == (nb if beg else na)
+        return cls(v_off, e_off, ptrs)
+
+    @classmethod
+    def from_device(cls, graphs: list[DeviceGraph]) -> "DeviceBatch":
+        from . import hiprt
+        sizes = [(g.V, g.E) for g in graphs]
+        v_off, e_off = cls._layout(sizes)
+        n = len(graphs)
+        nb, na = int(v_off[-1] + n) * 4, int(e_off[-1]) * 4
+        ptrs = [hiprt.malloc(nb), hiprt.malloc(na), hiprt.malloc(nb), hiprt.malloc(na)]
+        for i, g in enumerate(graphs):
+            bo, eo = int(v_off[i] + i) * 4, int(e_off[i]) * 4
+            hiprt.device_copy(ptrs[0] + bo, g.fw_beg, (g.V + 1) * 4)
+            hiprt.device_copy(ptrs[2] + bo, g.bw_beg, (g.V + 1) * 4)
+            if g.E:
+                hiprt.device_copy(ptrs[1] + eo, g.fw_adj, g.E * 4)
+                hiprt.device_copy(ptrs[3] + eo, g.bw_adj, g.E * 4)
+        return cls(v_off, e_off, ptrs)
+
+    def free(self):
+        from . import hiprt
+        for p in (self.fw_beg, self.fw_adj, self.bw_beg, self.bw_adj):
+            if p:
+                hiprt.free(p)
+        self.fw_beg = self.fw_adj = self.bw_beg = self.bw_adj = 0
 
 
 def scc_detection(g: HostGraph, alpha: int = 30, beta: int = 200, gamma: int = 10, theta: float = 0.1,

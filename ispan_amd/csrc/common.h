@@ -60,12 +60,12 @@ struct alignas(128) GLine {
     int32_t pad[31];
 };
 
-// Reduction carried by one grid barrier: two sums and a max over keys
-// (identity 0 for all three).
+// Reduction carried by one grid barrier: three sums and a max over keys
+// (identity 0 for all four).
 struct alignas(128) RLine {
-    int64_t sum[2];
+    int64_t sum[3];
     int64_t key;
-    int64_t pad[13];
+    int64_t pad[12];
 };
 
 // Device state of one k_giant launch; zeroed before every launch.
@@ -127,6 +127,13 @@ struct GiantArgs {
     int32_t alpha, beta, gamma;
     double avg_degree;
     int32_t wrap32;
+    // batched solve over a disjoint union of graphs: every graph's pivot
+    // (-1 entries are skipped) seeds both traversals; giant vertices keep
+    // their own id as label, to be replaced per graph afterwards
+    const int32_t* roots;
+    int32_t n_roots;          // 0: single graph, pivot chosen in the kernel
+    int32_t switch_rule;      // top-down -> bottom-up estimate of the frontier's work: 0 found x average
+                              // degree (the reference's), 1 the frontier's exact out-degree sum
 };
 
 constexpr int kGiantSlots = 16;
@@ -136,6 +143,15 @@ inline int64_t giant_hub_capacity(int64_t E) { return 5 * (E / kGiantChunk) + 64
 
 int giant_max_blocks_per_cu();
 hipError_t launch_giant(const GiantArgs& a, int grid, hipStream_t s);
+// batched solves (batch.hip)
+hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, const int32_t* bwb_cat,
+                              const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,
+                              int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
+                              int32_t* ubwa, int32_t* bad, hipStream_t s);
+hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const int64_t* goff, int32_t ng, int32_t wrap32,
+                              int32_t* roots, hipStream_t s);
+hipError_t launch_batch_finish(const int32_t* lab, const uint64_t* bvis, const int64_t* goff, const int32_t* roots,
+                               int32_t ng, int64_t Vu, int32_t* gmin, int32_t* out, hipStream_t s);
 hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s);
 
 // Grid sizes for grid-stride kernels.

@@ -33,7 +33,7 @@ struct Smem {
     int tmp[kW];
     int64_t red[kW];
     int64_t red2[kW];
-    int64_t rres[3];  // reduction result of the last barrier
+    int64_t rres[4];  // reduction result of the last barrier
     int32_t bi[kW];
     int ok;
     int32_t wbuf[kW][kWaveBuf];
@@ -52,7 +52,7 @@ struct Ctx {
 // Values a block contributes to the reduction a barrier carries: two sums
 // and a max over non-negative keys (0 contributes nothing).
 struct Red {
-    int64_t s0, s1, key;
+    int64_t s0, s1, s2, key;
 };
 
 // Two-level grid barrier: blocks arrive on one of 32 counters, the last of
@@ -72,7 +72,7 @@ struct Red {
 //
 // Returns false (and flags an error) if a wait exceeds kSpinCap.
 template <bool RED = false>
-__device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg, Red in = Red{0, 0, 0},
+__device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg, Red in = Red{0, 0, 0, 0},
                                       Red* out = nullptr) {
     using ull = unsigned long long;
     __syncthreads();
@@ -87,6 +87,7 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
         if (RED) {
             if (in.s0) atomicAdd(reinterpret_cast<ull*>(&ga.sum[0]), (ull)in.s0);
             if (in.s1) atomicAdd(reinterpret_cast<ull*>(&ga.sum[1]), (ull)in.s1);
+            if (in.s2) atomicAdd(reinterpret_cast<ull*>(&ga.sum[2]), (ull)in.s2);
             if (in.key) atomicMax(reinterpret_cast<ull*>(&ga.key), (ull)in.key);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -96,9 +97,11 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
             if (RED) {
                 const ull a0 = atomicExch(reinterpret_cast<ull*>(&ga.sum[0]), 0ull);
                 const ull a1 = atomicExch(reinterpret_cast<ull*>(&ga.sum[1]), 0ull);
+                const ull a2 = atomicExch(reinterpret_cast<ull*>(&ga.sum[2]), 0ull);
                 const ull ak = atomicExch(reinterpret_cast<ull*>(&ga.key), 0ull);
                 if (a0) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[0]), a0);
                 if (a1) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[1]), a1);
+                if (a2) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[2]), a2);
                 if (ak) atomicMax(reinterpret_cast<ull*>(&gs->tacc.key), ak);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -108,6 +111,7 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
                     RLine& r = gs->res[(g0 + 1) & 1];
                     st_agent(&r.sum[0], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[0]), 0ull));
                     st_agent(&r.sum[1], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[1]), 0ull));
+                    st_agent(&r.sum[2], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[2]), 0ull));
                     st_agent(&r.key, (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.key), 0ull));
                 }
                 last = true;
@@ -132,13 +136,14 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
             const RLine& r = gs->res[(g0 + 1) & 1];
             sm.rres[0] = load_agent(&r.sum[0]);
             sm.rres[1] = load_agent(&r.sum[1]);
-            sm.rres[2] = load_agent(&r.key);
+            sm.rres[2] = load_agent(&r.sum[2]);
+            sm.rres[3] = load_agent(&r.key);
         }
         sm.ok = ok && load_agent(&gs->err.v) == 0;
         if (ok) dbg[G + blockIdx.x] += 1;  // barriers passed
     }
     __syncthreads();
-    if (RED && out) *out = Red{sm.rres[0], sm.rres[1], sm.rres[2]};
+    if (RED && out) *out = Red{sm.rres[0], sm.rres[1], sm.rres[2], sm.rres[3]};
     __syncthreads();
     return sm.ok;
 }
@@ -216,21 +221,24 @@ __device__ __forceinline__ void hub_push(HubStage& hs, bool pred, int32_t v, int
 // the queue (degree-0 vertices have nothing to expand).  All lanes call it.
 __device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* __restrict__ beg, WaveQueue& wq,
                                           int32_t* qout, int32_t* qn_out, HubStage& hs, HubItem* hout,
-                                          int32_t* hn_out) {
+                                          int32_t* hn_out, int64_t& pushed_edges) {
     int32_t d = 0, s0 = 0;
     if (claim) {
         s0 = beg[w];
         d = beg[w + 1] - s0;
+        pushed_edges += d;
     }
     hub_push(hs, claim && d >= kGiantHubDeg, w, s0, d, hout, hn_out);
     wq_push<true>(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
 }
 
 // One direction of the traversal.  Returns the number of vertices claimed
-// (the root excluded), or -1 after a failed barrier.  BW: claims are limited
-// to the forward set and *gmin receives the smallest id reached.
+// (the roots excluded), or -1 after a failed barrier.  BW: claims are limited
+// to the forward set and *gmin receives the smallest id reached.  With
+// a.n_roots > 0 every listed root seeds level 0 (*seeded_out of them).
 template <bool BW>
-__device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* levels_out, int32_t* gmin_out) {
+__device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* levels_out,
+                                            int32_t* gmin_out, int64_t* seeded_out) {
     const GiantArgs& a = c.a;
     Smem& sm = c.sm;
     const int32_t* beg = BW ? a.bwb : a.fwb;  // expansion (top-down) direction
@@ -252,6 +260,28 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
     const int64_t total_waves = (int64_t)c.G * kW;
     const int64_t gwave = (int64_t)c.b * kW + c.wave;
     auto is_vis = [vis](int32_t u, int32_t) { return bit_of(vis, u); };
+    const bool multi = a.n_roots > 0;
+
+    int64_t fedges = 0;  // per thread: out-degree sum of what this thread pushed for the next level
+    *seeded_out = 1;
+    if (multi) {
+        // level-0 input: every graph's root, marked visited, queued or split
+        // into hub items like any claimed vertex
+        int64_t seeded = 0;
+        for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < a.n_roots; ib += (int64_t)c.G * kGBlock) {
+            const int64_t i = ib + c.lane;
+            const int32_t r = i < a.n_roots ? a.roots[i] : -1;
+            const bool ok = r >= 0 && r < a.V && (!BW || bit_of(a.fvis, r));
+            if (ok) atomicOr(reinterpret_cast<unsigned long long*>(vis + (r >> 6)), 1ull << (r & 63));
+            seeded += ok;
+            push_next(ok, r, beg, wq, a.q0, &gs->qn[dir][0].v, hs, a.h0, &gs->hn[dir][0].v, fedges);
+        }
+        wq_flush<true>(wq, a.q0, &gs->qn[dir][0].v);
+        hub_flush(hs, a.h0, &gs->hn[dir][0].v);
+        Red r;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{block_sum(seeded, sm), 0, 0, 0}, &r)) return -1;
+        *seeded_out = r.s0;
+    }
 
     while (true) {
         const int32_t* qin = (L & 1) ? a.q1 : a.q0;
@@ -266,12 +296,13 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         }
         const bool topdown = mode != 1;
         const int64_t edges0 = c.edges_td + c.edges_bu;  // thread 0's are the block's
+        fedges = 0;
         int64_t my_found = 0;  // per thread
         if (topdown) {
             // ---- top-down: tiles of the queue, then hub work items; level 0
             // expands the root's list in kGiantChunk-edge pieces.
             int64_t n = 0, nh = 0;
-            if (L == 0) {
+            if (L == 0 && !multi) {
                 if (c.b == 0 && c.tid == 0)
                     atomicOr(reinterpret_cast<unsigned long long*>(vis + (root >> 6)), 1ull << (root & 63));
                 nh = (beg[root + 1] - beg[root] + kGiantChunk - 1) / kGiantChunk;
@@ -317,12 +348,12 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         const bool claim = wave_try_claim(want, w, allow, vis);
                         my_found += claim;
                         if (BW && claim) gmin = min(gmin, w);
-                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out);
+                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out, fedges);
                     }
                     __syncthreads();
                 } else {
                     int32_t v, s0;
-                    if (L == 0) {
+                    if (L == 0 && !multi) {
                         v = root;
                         s0 = beg[root] + (int32_t)(u - ntiles) * kGiantChunk;
                     } else {
@@ -342,7 +373,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         const bool claim = wave_try_claim(want, w, allow, vis);
                         my_found += claim;
                         if (BW && claim) gmin = min(gmin, w);
-                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out);
+                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out, fedges);
                     }
                 }
             }
@@ -414,8 +445,9 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             bm = __shfl(bm, 0, kWave);
             key = (int64_t)INT32_MAX - bm;
         }
+        const int64_t blk_fe = block_sum(fedges, sm);
         Red r;
-        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{blk, c.edges_td + c.edges_bu - edges0, key}, &r)) return -1;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{blk, c.edges_td + c.edges_bu - edges0, blk_fe, key}, &r)) return -1;
         const int64_t found = r.s0;
         if (c.b == 0 && c.tid == 0 && L < kLevelTrace) {
             gs->lv_ts[dir][L] = wall_clock64();
@@ -424,6 +456,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             gs->lv_mode[dir][L] = topdown ? 0 : 1;
         }
         if (BW) *gmin_out = (int32_t)((int64_t)INT32_MAX - r.key);
+        const int64_t next_edges = r.s2;  // top-down levels: exact work of the next level
         visited += found;
         ++L;
         if (found == 0) break;
@@ -432,7 +465,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             return -1;
         }
         if (mode == 0) {
-            const double ef = (double)found * a.avg_degree;
+            const double ef = a.switch_rule ? (double)next_edges : (double)found * a.avg_degree;
             const double er = (double)(set_size - visited) * a.avg_degree;
             if (er / a.alpha < ef) mode = 1;
         } else if (mode == 1 && ((set_size * 1.0 / a.beta) > found || L - 1 > a.gamma)) {
@@ -453,7 +486,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     busy &= busy - 1;
                     const uint64_t m = __shfl(nw, l, kWave);
                     const int32_t v = (int32_t)((wb + l) * 64 + c.lane);
-                    push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next);
+                    push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next, fedges);
                 }
             }
             wq_flush<true>(wq, q_next, qn_next);
@@ -527,7 +560,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
             __hip_atomic_store(&gs->fqn[(s + 1) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int64_t t = trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v);
         Red r;
-        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{t, 0, 0}, &r)) return false;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{t, 0, 0, 0}, &r)) return false;
         trim1 += r.s0;
         n = load_agent(&gs->fqn[s % 3].v);
         cur ^= 1;
@@ -562,7 +595,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
         }
         const int64_t b2 = block_sum(c2, c.sm), b3 = block_sum(c3, c.sm);
         Red r;
-        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{b2, b3, 0}, &r)) return false;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{b2, b3, 0, 0}, &r)) return false;
         p2 = r.s0;
         p3 = r.s1;
         // P11: one more trim-1 sweep
@@ -714,19 +747,22 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         }
     __syncthreads();
     const int64_t fq0 = sum_slot(a.part, kSurv, c.G, sm);
-    const int32_t root = (bestp > 0 && besti < a.V) ? besti : -1;
+    const int32_t root = (a.n_roots == 0 && bestp > 0 && besti < a.V) ? besti : -1;
     if (stamp) gs->ts[1] = wall_clock64();
 
     int64_t fw_count = 0, giant = 0, fw_levels = 0, bw_levels = 0;
     int32_t gmin = root;
-    if (root >= 0) {
-        const int64_t fwc = traverse<false>(c, root, fq0, &fw_levels, &gmin);
+    const bool multi = a.n_roots > 0;
+    const bool traversed = multi || root >= 0;
+    if (traversed) {
+        int64_t seeded = 0;
+        const int64_t fwc = traverse<false>(c, root, fq0, &fw_levels, &gmin, &seeded);
         if (fwc < 0) return;
-        fw_count = fwc + 1;
+        fw_count = fwc + seeded;
         if (stamp) gs->ts[2] = wall_clock64();
-        const int64_t bwc = traverse<true>(c, root, fw_count, &bw_levels, &gmin);
+        const int64_t bwc = traverse<true>(c, root, fw_count, &bw_levels, &gmin, &seeded);
         if (bwc < 0) return;
-        giant = bwc + 1;
+        giant = bwc + seeded;
     }
     if (stamp) gs->ts[3] = wall_clock64();
 
@@ -738,11 +774,11 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     int64_t cnt = 0;
     for (int64_t w = wb + c.wave; w < we; w += kW) {
         const uint64_t a0 = load_agent(&a.act[w]);
-        const uint64_t g = root >= 0 ? load_agent(&a.bvis[w]) : 0ull;
+        const uint64_t g = traversed ? load_agent(&a.bvis[w]) : 0ull;
         const int64_t v = w * 64 + c.lane;
         if (v < a.V) {
             const bool gi = (g >> c.lane) & 1ull, ac = (a0 >> c.lane) & 1ull;
-            st_agent(&a.lab[v], gi ? gmin : (ac ? -1 : (int32_t)v));
+            st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (ac ? -1 : (int32_t)v));
         }
         if (c.lane == 0) {
             if (g) st_agent(&a.act[w], a0 & ~g);

@@ -76,6 +76,26 @@ struct ispan_ctx {
     GiantState* h_gstate = nullptr;  // pinned
     hipEvent_t ev[16] = {};
     bool have_solution = false;
+    // batched solves: consecutive graphs grouped so that each group's
+    // disjoint union fits int32 offsets; each group owns its union CSR
+    struct BatchGroup {
+        int32_t g0 = 0, ng = 0;
+        int64_t vbase = 0, ebase = 0;
+        int32_t V = 0;
+        int64_t E = 0;
+        int32_t* csr[4] = {nullptr, nullptr, nullptr, nullptr};  // fw_beg, fw_adj, bw_beg, bw_adj
+        int32_t* first[2] = {nullptr, nullptr};
+        int64_t* goff = nullptr;  // graph offsets inside the union [ng + 1]
+        int32_t* roots = nullptr;
+        int32_t* gmin = nullptr;
+    };
+    std::vector<BatchGroup> groups;
+    int32_t batch_n = -1;
+    std::vector<int64_t> batch_v_off;
+    int32_t* batch_labels = nullptr;  // [sum V_g], each graph's local canonical ids
+    bool batch_solved = false;
+    const int32_t* batch_roots = nullptr;  // set while one group is solved
+    int32_t batch_nroots = 0;
     // kernel profiling (params.timing >= 2)
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -142,7 +162,25 @@ void free_solve_buffers(ispan_ctx* ctx) {
     ctx->have_solution = false;
 }
 
+void free_batch(ispan_ctx* ctx) {
+    for (auto& g : ctx->groups) {
+        for (auto& p : g.csr) dfree(p);
+        for (auto& p : g.first) dfree(p);
+        dfree(g.goff);
+        dfree(g.roots);
+        dfree(g.gmin);
+    }
+    ctx->groups.clear();
+    dfree(ctx->batch_labels);
+    ctx->batch_n = -1;
+    ctx->batch_v_off.clear();
+    ctx->batch_solved = false;
+    ctx->batch_roots = nullptr;
+    ctx->batch_nroots = 0;
+}
+
 void free_graph(ispan_ctx* ctx) {
+    free_batch(ctx);
     for (auto& p : ctx->owned) dfree(p);
     dfree(ctx->fw_first);
     dfree(ctx->bw_first);
@@ -385,6 +423,9 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.gamma = P->gamma;
         ga.avg_degree = avg_degree;
         ga.wrap32 = P->pivot_wrap32;
+        ga.switch_rule = P->switch_rule;
+        ga.roots = ctx->batch_roots;
+        ga.n_roots = ctx->batch_nroots;
         ga.rest = 1;
         ga.trim_times = P->trim_times;
         ga.theta = P->theta;
@@ -448,8 +489,9 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             if (ctx->clock_khz > 0) {
                 const double k = 1.0 / ctx->clock_khz;  // ms per tick
                 giant_phase_ms[0] = (gs.ts[1] - gs.ts[0]) * k;
-                giant_phase_ms[1] = root >= 0 ? (gs.ts[2] - gs.ts[1]) * k : 0.0;
-                giant_phase_ms[2] = (gs.ts[4] - (root >= 0 ? gs.ts[2] : gs.ts[1])) * k;
+                const bool traversed = root >= 0 || ctx->batch_nroots > 0;
+                giant_phase_ms[1] = traversed ? (gs.ts[2] - gs.ts[1]) * k : 0.0;
+                giant_phase_ms[2] = (gs.ts[4] - (traversed ? gs.ts[2] : gs.ts[1])) * k;
                 giant_phase_ms[3] = gs.ts[5] > gs.ts[4] ? (gs.ts[5] - gs.ts[4]) * k : 0.0;
             }
             if (ctx->level_trace && ctx->clock_khz > 0) {
@@ -472,6 +514,10 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             ctx->prof.found[2] = gs.work[3];
             ctx->prof.found[3] = fq0;
         }
+    }
+    if (!coop && ctx->batch_nroots > 0) {
+        ctx->err = "batched solves need the cooperative engine (the device refused a cooperative launch)";
+        return ISPAN_ERR_HIP;
     }
     if (!coop) {
         // P1-P3: trim-1-first, active queue, pivot
@@ -714,6 +760,7 @@ void ispan_default_params(ispan_params* p) {
     p->validate = 1;
     p->async_batch = 8;
     p->engine = 0;
+    p->switch_rule = 1;
 }
 
 const char* ispan_strerror(int code) {
@@ -899,6 +946,219 @@ int ispan_solve_stats(ispan_ctx* ctx, ispan_stats* st) {
     st->bw_edges = ctx->h_ctl->bw_edges;
     st->n_scc = ctx->h_ctl->n_scc;
     return ISPAN_OK;
+}
+
+namespace {
+
+void use_group(ispan_ctx* ctx, const ispan_ctx::BatchGroup& g) {
+    ctx->fwb = g.csr[0];
+    ctx->fwa = g.csr[1];
+    ctx->bwb = g.csr[2];
+    ctx->bwa = g.csr[3];
+    ctx->fw_first = g.first[0];
+    ctx->bw_first = g.first[1];
+    ctx->V = g.V;
+    ctx->E = g.E;
+    ctx->nwords = ((int64_t)g.V + 63) / 64;
+    ctx->batch_roots = g.roots;
+    ctx->batch_nroots = g.ng;
+}
+
+void drop_group_view(ispan_ctx* ctx) {
+    ctx->fwb = ctx->fwa = ctx->bwb = ctx->bwa = nullptr;
+    ctx->fw_first = ctx->bw_first = nullptr;
+    ctx->V = -1;
+    ctx->E = 0;
+    ctx->nwords = 0;
+    ctx->batch_roots = nullptr;
+    ctx->batch_nroots = 0;
+}
+
+// Builds every group of a batch; on failure the caller frees the batch.
+int batch_build(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* e_off, const int32_t* fwb_cat,
+                const int32_t* fwa_cat, const int32_t* bwb_cat, const int32_t* bwa_cat, int64_t cap,
+                const ispan_params* P) {
+    hipStream_t s = ctx->stream;
+    int64_t* d_off = nullptr;
+    int32_t* d_bad = nullptr;
+    int rc = ISPAN_OK;
+    int32_t maxV = 0;
+    int64_t maxE = 0;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == ISPAN_OK) {
+            ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+            rc = e == hipErrorOutOfMemory ? ISPAN_ERR_NOMEM : ISPAN_ERR_HIP;
+        }
+        return rc == ISPAN_OK;
+    };
+    if (!hip(hipMalloc(&d_off, (size_t)2 * (n + 1) * 8), "hipMalloc") ||
+        !hip(hipMalloc(&d_bad, 4), "hipMalloc") ||
+        !hip(hipMemcpyAsync(d_off, v_off, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
+        !hip(hipMemcpyAsync(d_off + n + 1, e_off, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
+        !hip(hipMemsetAsync(d_bad, 0, 4, s), "hipMemsetAsync")) {
+        dfree(d_off);
+        dfree(d_bad);
+        return rc;
+    }
+    for (int32_t g0 = 0; g0 < n && rc == ISPAN_OK;) {
+        int32_t g1 = g0 + 1;
+        while (g1 < n && e_off[g1 + 1] - e_off[g0] <= cap && v_off[g1 + 1] - v_off[g0] <= INT32_MAX - 64) ++g1;
+        ctx->groups.emplace_back();
+        ispan_ctx::BatchGroup& G = ctx->groups.back();
+        G.g0 = g0;
+        G.ng = g1 - g0;
+        G.vbase = v_off[g0];
+        G.ebase = e_off[g0];
+        G.V = (int32_t)(v_off[g1] - v_off[g0]);
+        G.E = e_off[g1] - e_off[g0];
+        maxV = std::max(maxV, G.V);
+        maxE = std::max(maxE, G.E);
+        const size_t nb = ((size_t)G.V + 1) * 4, na = (size_t)std::max<int64_t>(G.E, 1) * 4;
+        std::vector<int64_t> goff(G.ng + 1);
+        for (int32_t i = 0; i <= G.ng; ++i) goff[i] = v_off[g0 + i] - G.vbase;
+        if (!hip(hipMalloc(&G.csr[0], nb), "hipMalloc") || !hip(hipMalloc(&G.csr[1], na), "hipMalloc") ||
+            !hip(hipMalloc(&G.csr[2], nb), "hipMalloc") || !hip(hipMalloc(&G.csr[3], na), "hipMalloc") ||
+            !hip(hipMalloc(&G.first[0], (size_t)std::max(G.V, 1) * 4), "hipMalloc") ||
+            !hip(hipMalloc(&G.first[1], (size_t)std::max(G.V, 1) * 4), "hipMalloc") ||
+            !hip(hipMalloc(&G.goff, goff.size() * 8), "hipMalloc") ||
+            !hip(hipMalloc(&G.roots, (size_t)G.ng * 4), "hipMalloc") ||
+            !hip(hipMalloc(&G.gmin, (size_t)G.ng * 4), "hipMalloc") ||
+            !hip(hipMemcpyAsync(G.goff, goff.data(), goff.size() * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
+            !hip(launch_batch_union(fwb_cat, fwa_cat, bwb_cat, bwa_cat, d_off, d_off + n + 1, g0, G.ng, G.V, G.E,
+                                    G.csr[0], G.csr[1], G.csr[2], G.csr[3], d_bad, s),
+                 "launch_batch_union"))
+            break;
+        int32_t bad = 0;
+        if (!hip(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") ||
+            !hip(hipStreamSynchronize(s), "hipStreamSynchronize"))
+            break;
+        if (bad) {
+            ctx->err = (bad & 1) ? "batch: a graph's offsets do not run from 0 to its edge count"
+                                 : "batch: an adjacency id lies outside its graph";
+            rc = ISPAN_ERR_CSR;
+            break;
+        }
+        use_group(ctx, G);
+        rc = check_bound(ctx, P);
+        drop_group_view(ctx);
+        if (rc) break;
+        // first arrays only once the offsets are known to be in range
+        if (G.V > 0 && (!hip(launch_first(G.csr[0], G.csr[1], G.V, G.first[0], ctx->grid, s), "launch_first") ||
+                        !hip(launch_first(G.csr[2], G.csr[3], G.V, G.first[1], ctx->grid, s), "launch_first")))
+            break;
+        g0 = g1;
+    }
+    if (rc == ISPAN_OK)
+        hip(hipMalloc(&ctx->batch_labels, (size_t)std::max<int64_t>(v_off[n], 1) * 4), "hipMalloc");
+    if (rc == ISPAN_OK) rc = ensure_buffers(ctx, maxV, maxE);
+    (void)hipStreamSynchronize(s);
+    dfree(d_off);
+    dfree(d_bad);
+    return rc;
+}
+
+void add_stats(ispan_stats& t, const ispan_stats& st) {
+    for (int k = 0; k < 16; ++k) t.ms[k] += st.ms[k];
+    t.fq0 += st.fq0;
+    t.fw_size += st.fw_size;
+    t.giant_size += st.giant_size;
+    t.trim1 += st.trim1;
+    t.trim2 += st.trim2;
+    t.trim3 += st.trim3;
+    t.n_wcc += st.n_wcc;
+    t.mice += st.mice;
+    t.fw_levels = std::max(t.fw_levels, st.fw_levels);
+    t.bw_levels = std::max(t.bw_levels, st.bw_levels);
+    t.host_syncs += st.host_syncs;
+}
+
+}  // namespace
+
+int ispan_batch_bind_device(ispan_ctx* ctx, int32_t n_graphs, const int64_t* h_v_off, const int64_t* h_e_off,
+                            const int32_t* d_fw_beg, const int32_t* d_fw_adj, const int32_t* d_bw_beg,
+                            const int32_t* d_bw_adj, int64_t max_group_edges, const ispan_params* params) {
+    if (!ctx || n_graphs < 1 || !h_v_off || !h_e_off || !d_fw_beg || !d_bw_beg || max_group_edges < 0)
+        return ISPAN_ERR_ARG;
+    if (h_v_off[0] != 0 || h_e_off[0] != 0) return ISPAN_ERR_ARG;
+    for (int32_t g = 0; g < n_graphs; ++g) {
+        const int64_t Vg = h_v_off[g + 1] - h_v_off[g], Eg = h_e_off[g + 1] - h_e_off[g];
+        if (Vg < 0 || Eg < 0 || Vg > INT32_MAX - 64 || Eg > INT32_MAX) return ISPAN_ERR_ARG;
+    }
+    if (h_e_off[n_graphs] > 0 && (!d_fw_adj || !d_bw_adj)) return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    free_graph(ctx);
+    ispan_params d;
+    ispan_default_params(&d);
+    const int64_t cap = max_group_edges > 0 ? std::min<int64_t>(max_group_edges, INT32_MAX) : INT32_MAX;
+    int rc = batch_build(ctx, n_graphs, h_v_off, h_e_off, d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj, cap,
+                         params ? params : &d);
+    if (rc) {
+        free_graph(ctx);
+        return rc;
+    }
+    ctx->batch_n = n_graphs;
+    ctx->batch_v_off.assign(h_v_off, h_v_off + n_graphs + 1);
+    return ISPAN_OK;
+}
+
+int ispan_batch_groups(const ispan_ctx* ctx) { return ctx && ctx->batch_n >= 0 ? (int)ctx->groups.size() : 0; }
+
+int ispan_batch_solve(ispan_ctx* ctx, const ispan_params* params, ispan_stats* stats) {
+    if (!ctx) return ISPAN_ERR_ARG;
+    if (ctx->batch_n < 0) return ISPAN_ERR_NOGRAPH;
+    ispan_params d;
+    ispan_default_params(&d);
+    const ispan_params* P = params ? params : &d;
+    if (P->alpha <= 0 || P->beta <= 0 || P->trim_times < 0) return ISPAN_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    const double t0 = wall_ms();
+    ispan_stats tot{};
+    ctx->batch_solved = false;
+    for (const auto& G : ctx->groups) {
+        if (G.V == 0) continue;
+        use_group(ctx, G);
+        int rc = ISPAN_OK;
+        hipError_t e = launch_batch_roots(G.csr[0], G.csr[2], G.goff, G.ng, P->pivot_wrap32, G.roots, ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("launch_batch_roots: ") + hipGetErrorString(e);
+            rc = ISPAN_ERR_HIP;
+        }
+        ispan_stats st{};
+        if (!rc) rc = solve_impl(ctx, P, nullptr, &st);
+        if (!rc) {
+            e = launch_batch_finish(ctx->lab, ctx->bvis, G.goff, G.roots, G.ng, G.V, G.gmin,
+                                    ctx->batch_labels + G.vbase, ctx->stream);
+            if (e != hipSuccess) {
+                ctx->err = std::string("launch_batch_finish: ") + hipGetErrorString(e);
+                rc = ISPAN_ERR_HIP;
+            }
+        }
+        drop_group_view(ctx);
+        ctx->have_solution = false;
+        if (rc) return rc;
+        add_stats(tot, st);
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    tot.ms[3] = wall_ms() - t0;
+    tot.pivot = -1;
+    tot.engine_used = 0;
+    ctx->batch_solved = true;
+    if (stats) *stats = tot;
+    return ISPAN_OK;
+}
+
+int ispan_batch_labels_download(ispan_ctx* ctx, int32_t* labels) {
+    if (!ctx || !labels) return ISPAN_ERR_ARG;
+    if (!ctx->batch_solved) return ISPAN_ERR_NOGRAPH;
+    const int64_t n = ctx->batch_v_off.back();
+    if (n == 0) return ISPAN_OK;
+    CK(hipMemcpyAsync(labels, ctx->batch_labels, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ISPAN_OK;
+}
+
+const int32_t* ispan_batch_labels_device(const ispan_ctx* ctx) {
+    return (ctx && ctx->batch_solved) ? ctx->batch_labels : nullptr;
 }
 
 int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,

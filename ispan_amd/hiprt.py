@@ -83,6 +83,11 @@ def device_copy(dst: int, src: int, nbytes: int):
     _ok(rt().hipMemcpy(dst, src, nbytes, _D2D), "hipMemcpy D2D")
 
 
+def upload(dst: int, host: np.ndarray):
+    h = np.ascontiguousarray(host)
+    _ok(rt().hipMemcpy(dst, h.ctypes.data, h.nbytes, _H2D), "hipMemcpy H2D")
+
+
 def malloc(nbytes: int) -> int:
     p = C.c_void_p()
     _ok(rt().hipMalloc(C.byref(p), max(nbytes, 1)), "hipMalloc")
