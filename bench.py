@@ -77,6 +77,67 @@ def giant_kernel_bytes(st, V: int) -> float:
     return 12.0 * V + 4.0 * st.fq0 + 4.0 * (st.fw_edges + st.bw_edges) + 12.0 * (st.fw_size + st.giant_size)
 
 
+def batch_leg(S, I, n_graphs: int, seed: int, steps: int, warmup: int) -> dict:
+    """Batched mode: n_graphs R-MAT graphs, V log-uniform in [1e3, 1e6], edge
+    factor uniform in [4, 16], (a,b,c,d) = (0.57,0.19,0.19,0.05), generated on
+    the device.  Timed: whole-batch solves (every group's launch, labels of
+    every graph on the device).  Compared with one solve per graph (each graph
+    bound on its own, only the solves timed); every graph's labels must match."""
+    rng = np.random.default_rng(seed)
+    Vs = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n_graphs)).astype(np.int64)
+    efs = rng.uniform(4.0, 16.0, n_graphs)
+    t0 = time.time()
+    dgs = []
+    out = {}
+    try:
+        for i in range(n_graphs):
+            dgs.append(S.gen_rmat(int(Vs[i]), float(efs[i]), seed * 1_000_003 + i))
+        sum_V, sum_E = sum(d.V for d in dgs), sum(d.E for d in dgs)
+        gen_s = time.time() - t0
+        log(f"[batch] {n_graphs} graphs, sum V={sum_V} sum E={sum_E}, generated in {gen_s:.1f}s")
+        t0 = time.time()
+        b = I.DeviceBatch.from_device(dgs)
+        try:
+            S.batch_bind(b, 0, I.default_params(validate=1))
+        finally:
+            b.free()
+        bind_s = time.time() - t0
+        p = I.default_params()
+        for _ in range(warmup):
+            S.batch_solve(p)
+        S.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = S.batch_solve(p)
+        S.synchronize()
+        batch_ms = (time.perf_counter() - t0) * 1e3 / steps
+        labs = S.batch_labels()
+        groups = S.batch_groups()
+        log(f"[batch] {groups} groups, {batch_ms:.3f} ms per batch")
+        # one graph per launch
+        single_ms, equal = 0.0, True
+        for dg, lab in zip(dgs, labs):
+            S.bind(dg, I.default_params(validate=0))
+            S.solve(p)  # warm
+            t1 = time.perf_counter()
+            S.solve(p)
+            single_ms += (time.perf_counter() - t1) * 1e3
+            equal = equal and bool(np.array_equal(S.labels(), lab))
+        out = {"graphs": n_graphs, "sum_V": int(sum_V), "sum_E": int(sum_E), "groups": groups,
+               "ms_per_batch": batch_ms, "graphs_per_s": n_graphs / (batch_ms * 1e-3),
+               "edges_per_s": sum_E / (batch_ms * 1e-3),
+               "one_per_launch": {"ms_total": single_ms, "graphs_per_s": n_graphs / (single_ms * 1e-3)},
+               "speedup_vs_one_per_launch": single_ms / batch_ms, "labels_equal_one_per_launch": equal,
+               "giant_total": int(st.giant_size), "gen_s": gen_s, "bind_s": bind_s,
+               "workload": f"{n_graphs} R-MAT graphs, V log-uniform [1e3, 1e6], EF uniform [4, 16], "
+                           f"(0.57,0.19,0.19,0.05), seeds {seed}*1000003+i, generated on the device"}
+        log(f"[batch] one per launch {single_ms:.1f} ms, speedup {single_ms / batch_ms:.1f}x, labels equal {equal}")
+    finally:
+        for dg in dgs:
+            dg.free()
+    return out
+
+
 def cpu_leg(dg, V: int, E: int, gpu_labels: np.ndarray, threads: int, runs: int, timeout: float) -> dict:
     import oracle as O
     t0 = time.time()
@@ -129,6 +190,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--timing", type=int, default=2, help="library timing level inside the timed region")
+    ap.add_argument("--batch-graphs", type=int, default=4096,
+                    help="batched leg: number of graphs (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -236,7 +299,13 @@ def main(argv=None) -> int:
                 d = json.load(f)
             result["roofline"]["traffic"] = d.get("hbm_bytes_per_launch")
             result["roofline"]["traffic_source"] = os.path.relpath(pmc, ROOT)
+    if a.batch_graphs > 0:
+        dg.free()
+        dg = None
+        result["batched"] = batch_leg(S, I, a.batch_graphs, a.seed, max(1, a.steps // 2), 1)
     if not a.no_cpu:
+        if dg is None:
+            dg, _ = build_graph(S, a.workload, a.seed)
         threads = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         cpu = cpu_leg(dg, V, E, labels, threads, a.cpu_runs, timeout=600)
         ref = cpu.get("reference")
@@ -251,7 +320,8 @@ def main(argv=None) -> int:
             "cpu_model": _cpu_model(), "detail": cpu}
         result["gpu_over_cpu"] = base["ms_mean"] / ms_per_step
         result["labels_equal_cpu"] = base["labels_equal_gpu"]
-    dg.free()
+    if dg is not None:
+        dg.free()
     S.close()
     print(json.dumps(result), flush=True)
     hiprt.release_device()
