@@ -40,6 +40,9 @@ extern "C" {
 #define ISPAN_ERR_UNLABELLED (-6) /* internal: a vertex left without a label     */
 #define ISPAN_ERR_IO (-7)         /* file missing, short or of the wrong size    */
 #define ISPAN_ERR_NOGRAPH (-8)    /* solve called before a graph was bound       */
+#define ISPAN_ERR_BUDGET (-9)     /* server: the HBM budget cannot admit it      */
+#define ISPAN_ERR_TIMEOUT (-10)   /* server: the wait timed out                  */
+#define ISPAN_ERR_NOTFOUND (-11)  /* server: unknown graph or request id         */
 
 /* Tuning knobs.  alpha/beta/gamma/theta keep the meaning of the `ispan`
  * argv (src/scc_main.cpp:20-25; used at src/fw_bw.h:581,599 and
@@ -185,6 +188,60 @@ int ispan_batch_solve(ispan_ctx* ctx, const ispan_params* params, ispan_stats* s
  * v_off[g], each the smallest local id of the vertex's SCC. */
 int ispan_batch_labels_download(ispan_ctx* ctx, int32_t* labels);
 const int32_t* ispan_batch_labels_device(const ispan_ctx* ctx);
+
+/* ---- serving --------------------------------------------------------------
+ * A request loop over device-resident graphs with continuous batching; no
+ * counterpart in the reference.  Graphs are registered once (copied to the
+ * device, checked, admitted against an HBM budget that also covers the
+ * working state of the largest batch).  A request names a graph.  A worker
+ * thread takes every pending request that fits the batch limits and the
+ * budget, solves them as one union (one cooperative launch) and starts over
+ * with what arrived meanwhile; label copies back to the host overlap the next
+ * batch's solve.  Each request's labels equal those of a single solve of its
+ * graph.  All functions are thread-safe. */
+typedef struct ispan_server ispan_server;
+typedef struct {
+    int64_t hbm_budget;       /* device bytes for resident graphs + batch state (0: 200 GiB) */
+    int32_t max_batch_graphs; /* requests per batch (0: no limit)                          */
+    int32_t reserved0;
+    int64_t max_batch_edges;  /* edges per batch (0: 2^31 - 1)                             */
+    int64_t reserved[5];
+} ispan_server_config;
+typedef struct {
+    double submit_ms, start_ms, done_ms; /* steady clock, ms                       */
+    double latency_ms;                   /* done - submit                          */
+    int64_t batch;                       /* batch that solved it                   */
+    int32_t batch_graphs;                /* requests in that batch                 */
+    int32_t status;                      /* ISPAN_OK or the batch's error          */
+} ispan_request_info;
+typedef struct {
+    int64_t requests_done, batches, graphs_resident, bytes_resident, bytes_budget;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency (submit -> labels on the host) */
+    double mean_batch_graphs;
+    int64_t reserved[6];
+} ispan_server_stats;
+
+int ispan_server_create(ispan_server** out, int device, const ispan_server_config* cfg, const ispan_params* params);
+/* Finishes every submitted request, then stops. */
+int ispan_server_destroy(ispan_server* srv);
+/* Register a graph from host (checked in full) or device memory (copied, then
+ * checked); ISPAN_ERR_BUDGET when it does not fit. */
+int ispan_server_add_graph(ispan_server* srv, int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj,
+                           const int32_t* bw_beg, const int32_t* bw_adj, int64_t* graph_id);
+int ispan_server_add_graph_device(ispan_server* srv, int32_t V, int64_t E, const int32_t* d_fw_beg,
+                                  const int32_t* d_fw_adj, const int32_t* d_bw_beg, const int32_t* d_bw_adj,
+                                  int64_t* graph_id);
+/* Unregister; the memory is released once its pending requests finish. */
+int ispan_server_remove_graph(ispan_server* srv, int64_t graph_id);
+/* Queue a solve of graph_id; returns at once. */
+int ispan_server_submit(ispan_server* srv, int64_t graph_id, int64_t* request_id);
+/* Wait for a request (timeout_ms < 0: no limit); labels: int32[V] or NULL.
+ * Returns the request's status; the request is forgotten afterwards. */
+int ispan_server_wait(ispan_server* srv, int64_t request_id, int32_t* labels, ispan_request_info* info,
+                      int64_t timeout_ms);
+int ispan_server_stats_get(ispan_server* srv, ispan_server_stats* out);
+int ispan_server_reset_stats(ispan_server* srv);
+const char* ispan_server_last_error(const ispan_server* srv);
 
 /* ---- reference file I/O and reporting ---------------------------------- */
 

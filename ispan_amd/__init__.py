@@ -27,7 +27,8 @@ CLI_PATH = os.path.join(PKG, "bin", "ispan")
 
 ISPAN_OK = 0
 ERRORS = {-1: "ISPAN_ERR_ARG", -2: "ISPAN_ERR_CSR", -3: "ISPAN_ERR_NOMEM", -4: "ISPAN_ERR_HIP",
-          -5: "ISPAN_ERR_ITER", -6: "ISPAN_ERR_UNLABELLED", -7: "ISPAN_ERR_IO", -8: "ISPAN_ERR_NOGRAPH"}
+          -5: "ISPAN_ERR_ITER", -6: "ISPAN_ERR_UNLABELLED", -7: "ISPAN_ERR_IO", -8: "ISPAN_ERR_NOGRAPH",
+          -9: "ISPAN_ERR_BUDGET", -10: "ISPAN_ERR_TIMEOUT", -11: "ISPAN_ERR_NOTFOUND"}
 
 # Every symbol include/ispan_hip.h declares.
 EXPORTS = [
@@ -37,7 +38,9 @@ EXPORTS = [
     "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
     "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
     "ispan_convert_edge_list", "ispan_batch_bind_device", "ispan_batch_groups", "ispan_batch_solve",
-    "ispan_batch_labels_download", "ispan_batch_labels_device",
+    "ispan_batch_labels_download", "ispan_batch_labels_device", "ispan_server_create", "ispan_server_destroy",
+    "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit",
+    "ispan_server_wait", "ispan_server_stats_get", "ispan_server_reset_stats", "ispan_server_last_error",
 ]
 
 
@@ -75,6 +78,28 @@ class Stats(C.Structure):
                  13: "trim_size_3", 14: "trim_1_sweeps"}
         d["ms"] = {n: self.ms[i] for i, n in names.items()}
         return d
+
+
+class ServerConfig(C.Structure):
+    """ispan_server_config (include/ispan_hip.h)."""
+    _fields_ = [("hbm_budget", C.c_int64), ("max_batch_graphs", C.c_int32), ("reserved0", C.c_int32),
+                ("max_batch_edges", C.c_int64), ("reserved", C.c_int64 * 5)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("submit_ms", C.c_double), ("start_ms", C.c_double), ("done_ms", C.c_double),
+                ("latency_ms", C.c_double), ("batch", C.c_int64), ("batch_graphs", C.c_int32),
+                ("status", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_int64), ("batches", C.c_int64), ("graphs_resident", C.c_int64),
+                ("bytes_resident", C.c_int64), ("bytes_budget", C.c_int64), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("mean_batch_graphs", C.c_double), ("reserved", C.c_int64 * 6)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class KernelProfile(C.Structure):
@@ -146,6 +171,17 @@ def lib():
                                              C.POINTER(C.c_int64)]),
         "ispan_device_free": (None, [_vp]),
         "ispan_kernel_profile_get": (C.c_int, [_vp, C.POINTER(KernelProfile)]),
+        "ispan_server_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(ServerConfig), C.POINTER(Params)]),
+        "ispan_server_destroy": (C.c_int, [_vp]),
+        "ispan_server_add_graph": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(C.c_int64)]),
+        "ispan_server_add_graph_device": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp,
+                                                    C.POINTER(C.c_int64)]),
+        "ispan_server_remove_graph": (C.c_int, [_vp, C.c_int64]),
+        "ispan_server_submit": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int64)]),
+        "ispan_server_wait": (C.c_int, [_vp, C.c_int64, _vp, C.POINTER(RequestInfo), C.c_int64]),
+        "ispan_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+        "ispan_server_reset_stats": (C.c_int, [_vp]),
+        "ispan_server_last_error": (C.c_char_p, [_vp]),
         "ispan_batch_bind_device": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int64,
                                               C.POINTER(Params)]),
         "ispan_batch_groups": (C.c_int, [_vp]),
@@ -394,6 +430,79 @@ class Solver:
         _check(lib().ispan_build_csr_device(self._ctx, V, m, d_src, d_dst, *[C.byref(p) for p in ptrs],
                                             C.byref(E)), self._ctx)
         return DeviceGraph(V, E.value, *[C.cast(p, C.c_void_p).value or 0 for p in ptrs])
+
+
+class Server:
+    """Continuous-batching SCC server (ispan_server_*): graphs resident on the
+    device, requests batched as they arrive."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch_graphs: int = 0, max_batch_edges: int = 0,
+                 params: Params | None = None):
+        cfg = ServerConfig(hbm_budget=hbm_budget, max_batch_graphs=max_batch_graphs,
+                           max_batch_edges=max_batch_edges)
+        h = C.c_void_p()
+        _check(lib().ispan_server_create(C.byref(h), device, C.byref(cfg), C.byref(params) if params else None))
+        self._srv = h.value
+        self._V = {}      # graph id -> V
+        self._rid_V = {}  # request id -> V of its graph
+
+    def _err(self, rc: int):
+        if rc != 0:
+            raise IspanError(rc, (lib().ispan_server_last_error(self._srv) or b"").decode())
+
+    def add_graph(self, g: "HostGraph | DeviceGraph") -> int:
+        gid = C.c_int64()
+        if isinstance(g, DeviceGraph):
+            rc = lib().ispan_server_add_graph_device(self._srv, g.V, g.E, g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj,
+                                                     C.byref(gid))
+            V = g.V
+        else:
+            arrs = [np.ascontiguousarray(a, np.int32) for a in (g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)]
+            V, E = len(arrs[0]) - 1, len(arrs[1])
+            rc = lib().ispan_server_add_graph(self._srv, V, E, *[_ptr(a) for a in arrs], C.byref(gid))
+        self._err(rc)
+        self._V[gid.value] = V
+        return gid.value
+
+    def remove_graph(self, gid: int):
+        self._err(lib().ispan_server_remove_graph(self._srv, gid))
+
+    def submit(self, gid: int) -> int:
+        rid = C.c_int64()
+        self._err(lib().ispan_server_submit(self._srv, gid, C.byref(rid)))
+        self._rid_V[rid.value] = self._V[gid]
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1, want_labels: bool = True):
+        """-> (labels or None, RequestInfo)."""
+        V = self._rid_V.pop(rid, 0)
+        out = np.empty(max(V, 1), np.int32) if want_labels else None
+        info = RequestInfo()
+        rc = lib().ispan_server_wait(self._srv, rid, _ptr(out) if out is not None else None, C.byref(info),
+                                     timeout_ms)
+        if rc == -10:
+            self._rid_V[rid] = V
+        self._err(rc)
+        return (out[:V] if out is not None else None), info
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        self._err(lib().ispan_server_stats_get(self._srv, C.byref(st)))
+        return st
+
+    def reset_stats(self):
+        self._err(lib().ispan_server_reset_stats(self._srv))
+
+    def close(self):
+        if self._srv:
+            lib().ispan_server_destroy(self._srv)
+            self._srv = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class DeviceBatch:

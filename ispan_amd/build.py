@@ -22,7 +22,7 @@ LIB = os.path.join(LIB_DIR, "libispan_hip.so")
 CLI = os.path.join(BIN_DIR, "ispan")
 
 ARCH = os.environ.get("ISPAN_OFFLOAD_ARCH", "gfx950")
-HIP_SOURCES = ["kernels.hip", "giant.hip", "solver.hip", "gen.hip", "batch.hip"]
+HIP_SOURCES = ["kernels.hip", "giant.hip", "solver.hip", "gen.hip", "batch.hip", "server.hip"]
 # -packed-fp32-ops: no packed FP32 instructions in any kernel.
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", f"-I{ROOT}/include", f"-I{CSRC}"]

@@ -30,14 +30,32 @@ __device__ __forceinline__ int32_t find_graph(const int64_t* off, int32_t n, int
     return lo;
 }
 
+// Where graph g's arrays are: back to back (the batch ABI's layout) or
+// through a table of per-graph device pointers (the server's resident graphs).
+struct CatSrc {
+    const int32_t* beg;
+    const int32_t* adj;
+    const int64_t* v_off;
+    const int64_t* e_off;
+    __device__ const int32_t* beg_of(int32_t g) const { return beg + v_off[g] + g; }
+    __device__ const int32_t* adj_of(int32_t g) const { return adj + e_off[g]; }
+};
+
+struct PtrSrc {
+    const int32_t* const* beg;
+    const int32_t* const* adj;
+    __device__ const int32_t* beg_of(int32_t g) const { return beg[g]; }
+    __device__ const int32_t* adj_of(int32_t g) const { return adj[g]; }
+};
+
 // Union offsets: ubeg[u] for the group's vertices u = v_off[g] - v_off[g0] + i
 // is (e_off[g] - e_off[g0]) + beg_g[i].  Blocks take contiguous chunks, so a
 // thread's graph index only moves forward.  bad |= 1: a graph's offsets do
 // not start at 0 / end at E_g, or one lies outside [0, E_g] (an offset out
 // of range is clamped, so no later kernel reads outside the adjacency).
-__global__ void k_union_beg(const int32_t* __restrict__ beg_cat, const int64_t* __restrict__ v_off,
-                            const int64_t* __restrict__ e_off, int32_t g0, int32_t ng, int32_t* __restrict__ ubeg,
-                            int32_t* bad) {
+template <class Src>
+__global__ void k_union_beg(Src src, const int64_t* __restrict__ v_off, const int64_t* __restrict__ e_off, int32_t g0,
+                            int32_t ng, int32_t* __restrict__ ubeg, int32_t* bad) {
     const int64_t vb = v_off[g0], Vu = v_off[g0 + ng] - vb, eb = e_off[g0];
     const int64_t c0 = (int64_t)blockIdx.x * kChunk;
     if (c0 >= Vu) return;
@@ -46,23 +64,24 @@ __global__ void k_union_beg(const int32_t* __restrict__ beg_cat, const int64_t* 
     for (int64_t u = c0 + threadIdx.x; u < c1; u += blockDim.x) {
         while (vb + u >= v_off[g + 1]) ++g;
         const int64_t i = vb + u - v_off[g];
-        const int64_t Eg = e_off[g + 1] - e_off[g];
-        int64_t b = beg_cat[v_off[g] + g + i];
+        const int64_t Vg = v_off[g + 1] - v_off[g], Eg = e_off[g + 1] - e_off[g];
+        const int32_t* bg = src.beg_of(g);
+        int64_t b = bg[i];
         if (b < 0 || b > Eg) {
             atomicOr(bad, 1);
             b = b < 0 ? 0 : Eg;
         }
         ubeg[u] = (int32_t)(e_off[g] - eb + b);
-        if (i == 0 && (b != 0 || (int64_t)beg_cat[v_off[g + 1] + g] != Eg)) atomicOr(bad, 1);
+        if (i == 0 && (b != 0 || (int64_t)bg[Vg] != Eg)) atomicOr(bad, 1);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ubeg[Vu] = (int32_t)(e_off[g0 + ng] - eb);
 }
 
 // Union adjacency: local ids shifted by the graph's vertex base.  bad |= 2: an
 // id outside [0, V_g).
-__global__ void k_union_adj(const int32_t* __restrict__ adj_cat, const int64_t* __restrict__ v_off,
-                            const int64_t* __restrict__ e_off, int32_t g0, int32_t ng, int32_t* __restrict__ uadj,
-                            int32_t* bad) {
+template <class Src>
+__global__ void k_union_adj(Src src, const int64_t* __restrict__ v_off, const int64_t* __restrict__ e_off, int32_t g0,
+                            int32_t ng, int32_t* __restrict__ uadj, int32_t* bad) {
     const int64_t vb = v_off[g0], eb = e_off[g0], Eu = e_off[g0 + ng] - eb;
     const int64_t c0 = (int64_t)blockIdx.x * kChunk;
     if (c0 >= Eu) return;
@@ -72,7 +91,7 @@ __global__ void k_union_adj(const int32_t* __restrict__ adj_cat, const int64_t* 
     int any_bad = 0;
     for (int64_t k = c0 + threadIdx.x; k < c1; k += blockDim.x) {
         while (eb + k >= e_off[g + 1]) ++g;
-        const int32_t w = adj_cat[eb + k];
+        const int32_t w = src.adj_of(g)[eb + k - e_off[g]];
         const int64_t Vg = v_off[g + 1] - v_off[g];
         any_bad |= (w < 0 || w >= Vg);
         uadj[k] = (int32_t)(w + (v_off[g] - vb));
@@ -181,21 +200,37 @@ int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kChunk - 1) / 
 
 }  // namespace
 
+template <class Src>
+hipError_t union_launch(Src fw, Src bw, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0, int32_t ng,
+                        int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb, int32_t* ubwa,
+                        int32_t* bad, hipStream_t s) {
+    hipLaunchKernelGGL(k_union_beg<Src>, dim3(blocks_for(Vu)), dim3(kBlock), 0, s, fw, d_v_off, d_e_off, g0, ng, ufwb,
+                       bad);
+    hipLaunchKernelGGL(k_union_beg<Src>, dim3(blocks_for(Vu)), dim3(kBlock), 0, s, bw, d_v_off, d_e_off, g0, ng, ubwb,
+                       bad);
+    if (Eu > 0) {
+        hipLaunchKernelGGL(k_union_adj<Src>, dim3(blocks_for(Eu)), dim3(kBlock), 0, s, fw, d_v_off, d_e_off, g0, ng,
+                           ufwa, bad);
+        hipLaunchKernelGGL(k_union_adj<Src>, dim3(blocks_for(Eu)), dim3(kBlock), 0, s, bw, d_v_off, d_e_off, g0, ng,
+                           ubwa, bad);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, const int32_t* bwb_cat,
                               const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,
                               int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
                               int32_t* ubwa, int32_t* bad, hipStream_t s) {
-    hipLaunchKernelGGL(k_union_beg, dim3(blocks_for(Vu)), dim3(kBlock), 0, s, fwb_cat, d_v_off, d_e_off, g0, ng, ufwb,
-                       bad);
-    hipLaunchKernelGGL(k_union_beg, dim3(blocks_for(Vu)), dim3(kBlock), 0, s, bwb_cat, d_v_off, d_e_off, g0, ng, ubwb,
-                       bad);
-    if (Eu > 0) {
-        hipLaunchKernelGGL(k_union_adj, dim3(blocks_for(Eu)), dim3(kBlock), 0, s, fwa_cat, d_v_off, d_e_off, g0, ng,
-                           ufwa, bad);
-        hipLaunchKernelGGL(k_union_adj, dim3(blocks_for(Eu)), dim3(kBlock), 0, s, bwa_cat, d_v_off, d_e_off, g0, ng,
-                           ubwa, bad);
-    }
-    return hipGetLastError();
+    return union_launch(CatSrc{fwb_cat, fwa_cat, d_v_off, d_e_off}, CatSrc{bwb_cat, bwa_cat, d_v_off, d_e_off}, d_v_off,
+                        d_e_off, g0, ng, Vu, Eu, ufwb, ufwa, ubwb, ubwa, bad, s);
+}
+
+hipError_t launch_batch_union_ptrs(const int32_t* const* fwb, const int32_t* const* fwa, const int32_t* const* bwb,
+                                   const int32_t* const* bwa, const int64_t* d_v_off, const int64_t* d_e_off,
+                                   int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
+                                   int32_t* ubwa, int32_t* bad, hipStream_t s) {
+    return union_launch(PtrSrc{fwb, fwa}, PtrSrc{bwb, bwa}, d_v_off, d_e_off, 0, ng, Vu, Eu, ufwb, ufwa, ubwb, ubwa,
+                        bad, s);
 }
 
 hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const int64_t* goff, int32_t ng, int32_t wrap32,

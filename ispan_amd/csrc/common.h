@@ -148,6 +148,10 @@ hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, co
                               const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,
                               int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
                               int32_t* ubwa, int32_t* bad, hipStream_t s);
+hipError_t launch_batch_union_ptrs(const int32_t* const* fwb, const int32_t* const* fwa, const int32_t* const* bwb,
+                                   const int32_t* const* bwa, const int64_t* d_v_off, const int64_t* d_e_off,
+                                   int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
+                                   int32_t* ubwa, int32_t* bad, hipStream_t s);
 hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const int64_t* goff, int32_t ng, int32_t wrap32,
                               int32_t* roots, hipStream_t s);
 hipError_t launch_batch_finish(const int32_t* lab, const uint64_t* bvis, const int64_t* goff, const int32_t* roots,

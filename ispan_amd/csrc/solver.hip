@@ -20,6 +20,7 @@
 
 #include "../../include/ispan_hip.h"
 #include "common.h"
+#include "internal.h"
 
 using namespace ispan;
 
@@ -96,6 +97,13 @@ struct ispan_ctx {
     bool batch_solved = false;
     const int32_t* batch_roots = nullptr;  // set while one group is solved
     int32_t batch_nroots = 0;
+    // reusable union workspace of batch_solve_tables (the server's batches)
+    BatchGroup ws;
+    int32_t ws_capV = -1, ws_capN = -1;
+    int64_t ws_capE = -1;
+    int64_t* ws_off = nullptr;             // v_off, e_off [2 * (capN + 1)]
+    const int32_t** ws_ptrs = nullptr;     // 4 pointer tables [4 * capN]
+    int32_t* ws_bad = nullptr;
     // kernel profiling (params.timing >= 2)
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -177,6 +185,20 @@ void free_batch(ispan_ctx* ctx) {
     ctx->batch_solved = false;
     ctx->batch_roots = nullptr;
     ctx->batch_nroots = 0;
+}
+
+void free_workspace(ispan_ctx* ctx) {
+    for (auto& p : ctx->ws.csr) dfree(p);
+    for (auto& p : ctx->ws.first) dfree(p);
+    dfree(ctx->ws.goff);
+    dfree(ctx->ws.roots);
+    dfree(ctx->ws.gmin);
+    dfree(ctx->ws_off);
+    if (ctx->ws_ptrs) (void)hipFree((void*)ctx->ws_ptrs);
+    ctx->ws_ptrs = nullptr;
+    dfree(ctx->ws_bad);
+    ctx->ws_capV = ctx->ws_capN = -1;
+    ctx->ws_capE = -1;
 }
 
 void free_graph(ispan_ctx* ctx) {
@@ -774,6 +796,9 @@ const char* ispan_strerror(int code) {
         case ISPAN_ERR_UNLABELLED: return "vertices left unlabelled";
         case ISPAN_ERR_IO: return "file I/O error";
         case ISPAN_ERR_NOGRAPH: return "no graph bound";
+        case ISPAN_ERR_BUDGET: return "HBM budget exceeded";
+        case ISPAN_ERR_TIMEOUT: return "timed out";
+        case ISPAN_ERR_NOTFOUND: return "unknown id";
         default: return "unknown error";
     }
 }
@@ -824,6 +849,7 @@ int ispan_ctx_destroy(ispan_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_solve_buffers(ctx);
     free_graph(ctx);
+    free_workspace(ctx);
     dfree(ctx->ctl);
     dfree(ctx->gstate);
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
@@ -1073,6 +1099,105 @@ void add_stats(ispan_stats& t, const ispan_stats& st) {
 }
 
 }  // namespace
+
+}  // extern "C"
+
+namespace ispan {
+
+int64_t batch_workspace_bytes(int64_t V, int64_t E) {
+    // union CSR pair + first arrays, and the per-vertex solve state
+    return 8 * E + 16 * (V + 1) + 80 * (V + 1) + 2 * giant_hub_capacity(E) * (int64_t)sizeof(HubItem);
+}
+
+int batch_solve_tables(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* e_off,
+                       const int32_t* const* fwb, const int32_t* const* fwa, const int32_t* const* bwb,
+                       const int32_t* const* bwa, const ispan_params* P, int32_t* d_out, ispan_stats* st) {
+    if (!ctx || n < 1 || v_off[0] != 0 || e_off[0] != 0) return ISPAN_ERR_ARG;
+    const int64_t Vu = v_off[n], Eu = e_off[n];
+    if (Vu > INT32_MAX - 64 || Eu > INT32_MAX) return ISPAN_ERR_ARG;
+    if (ctx->batch_n >= 0 || ctx->V >= 0) free_graph(ctx);
+    hipStream_t s = ctx->stream;
+    ispan_ctx::BatchGroup& G = ctx->ws;
+    if (Vu > ctx->ws_capV || Eu > ctx->ws_capE) {
+        for (auto& p : G.csr) dfree(p);
+        for (auto& p : G.first) dfree(p);
+        const size_t nb = ((size_t)Vu + 1) * 4, na = (size_t)std::max<int64_t>(Eu, 1) * 4;
+        CK(hipMalloc(&G.csr[0], nb));
+        CK(hipMalloc(&G.csr[1], na));
+        CK(hipMalloc(&G.csr[2], nb));
+        CK(hipMalloc(&G.csr[3], na));
+        CK(hipMalloc(&G.first[0], nb));
+        CK(hipMalloc(&G.first[1], nb));
+        ctx->ws_capV = (int32_t)Vu;
+        ctx->ws_capE = Eu;
+    }
+    if (n > ctx->ws_capN) {
+        dfree(G.goff);
+        dfree(G.roots);
+        dfree(G.gmin);
+        dfree(ctx->ws_off);
+        if (ctx->ws_ptrs) (void)hipFree((void*)ctx->ws_ptrs);
+        ctx->ws_ptrs = nullptr;
+        CK(hipMalloc(&G.goff, ((size_t)n + 1) * 8));
+        CK(hipMalloc(&G.roots, (size_t)n * 4));
+        CK(hipMalloc(&G.gmin, (size_t)n * 4));
+        CK(hipMalloc(&ctx->ws_off, 2 * ((size_t)n + 1) * 8));
+        CK(hipMalloc((void**)&ctx->ws_ptrs, 4 * (size_t)n * sizeof(void*)));
+        if (!ctx->ws_bad) CK(hipMalloc(&ctx->ws_bad, 4));
+        ctx->ws_capN = n;
+    }
+    G.g0 = 0;
+    G.ng = n;
+    G.vbase = 0;
+    G.ebase = 0;
+    G.V = (int32_t)Vu;
+    G.E = Eu;
+    std::vector<const int32_t*> tabs(4 * (size_t)n);
+    for (int32_t g = 0; g < n; ++g) {
+        tabs[g] = fwb[g];
+        tabs[n + g] = fwa[g];
+        tabs[2 * (size_t)n + g] = bwb[g];
+        tabs[3 * (size_t)n + g] = bwa[g];
+    }
+    CK(hipMemcpyAsync(ctx->ws_off, v_off, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(ctx->ws_off + n + 1, e_off, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(G.goff, v_off, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync((void*)ctx->ws_ptrs, tabs.data(), tabs.size() * sizeof(void*), hipMemcpyHostToDevice, s));
+    // the graphs were checked when they were registered
+    CK(launch_batch_union_ptrs(ctx->ws_ptrs, ctx->ws_ptrs + n, ctx->ws_ptrs + 2 * (size_t)n,
+                               ctx->ws_ptrs + 3 * (size_t)n, ctx->ws_off, ctx->ws_off + n + 1, n, Vu, Eu, G.csr[0],
+                               G.csr[1], G.csr[2], G.csr[3], ctx->ws_bad, s));
+    if (Vu > 0) {
+        CK(launch_first(G.csr[0], G.csr[1], (int32_t)Vu, G.first[0], ctx->grid, s));
+        CK(launch_first(G.csr[2], G.csr[3], (int32_t)Vu, G.first[1], ctx->grid, s));
+    }
+    ispan_stats local{};
+    int rc = ISPAN_OK;
+    if (Vu > 0) {
+        use_group(ctx, G);
+        hipError_t e = launch_batch_roots(G.csr[0], G.csr[2], G.goff, n, P->pivot_wrap32, G.roots, s);
+        if (e != hipSuccess) {
+            ctx->err = std::string("launch_batch_roots: ") + hipGetErrorString(e);
+            rc = ISPAN_ERR_HIP;
+        }
+        if (!rc) rc = solve_impl(ctx, P, nullptr, &local);
+        if (!rc) {
+            e = launch_batch_finish(ctx->lab, ctx->bvis, G.goff, G.roots, n, Vu, G.gmin, d_out, s);
+            if (e != hipSuccess) {
+                ctx->err = std::string("launch_batch_finish: ") + hipGetErrorString(e);
+                rc = ISPAN_ERR_HIP;
+            }
+        }
+        drop_group_view(ctx);
+        ctx->have_solution = false;
+    }
+    if (st) *st = local;
+    return rc;
+}
+
+}  // namespace ispan
+
+extern "C" {
 
 int ispan_batch_bind_device(ispan_ctx* ctx, int32_t n_graphs, const int64_t* h_v_off, const int64_t* h_e_off,
                             const int32_t* d_fw_beg, const int32_t* d_fw_adj, const int32_t* d_bw_beg,

@@ -1,0 +1,129 @@
+"""GPU tests of the continuous-batching server: every request's labels are the
+same solved alone and in a mix of concurrent requests, and equal Tarjan's or
+a single solve's; batching happens; the HBM budget gates registration."""
+import threading
+
+import numpy as np
+import pytest
+
+import ispan_amd as I
+import oracle as O
+from tests import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def host(g):
+    return I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj)
+
+
+@pytest.fixture(scope="module")
+def pool():
+    graphs = [O.rmat(s, 16, 40 + s) for s in (6, 9, 11, 12, 13)]
+    graphs += [G.cycles_graph(5, n_cycles=60), G.random_graph(3, vmax=800), O.grid(40, 0.6, 2)]
+    graphs += [G.pattern_graph(d) for _, d in G.pattern_cases()[:3]]
+    graphs += [O.build_csr(3, np.zeros(0, np.int32), np.zeros(0, np.int32))]
+    return graphs, [O.tarjan(g) for g in graphs]
+
+
+def test_alone_and_in_mix(pool):
+    graphs, ref = pool
+    with I.Server(0) as srv:
+        ids = [srv.add_graph(host(g)) for g in graphs]
+        alone = []
+        for gid in ids:  # one at a time: every batch holds one request
+            lab, info = srv.wait(srv.submit(gid))
+            assert info.status == 0 and info.batch_graphs == 1
+            alone.append(lab)
+        for lab, r in zip(alone, ref):
+            assert np.array_equal(lab, r)
+        # four client threads, each submitting the whole pool three times
+        results, errors = [], []
+
+        def client(seed):
+            try:
+                order = np.random.default_rng(seed).permutation(np.repeat(np.arange(len(ids)), 3))
+                rids = [(int(k), srv.submit(ids[int(k)])) for k in order]
+                for k, rid in rids:
+                    lab, info = srv.wait(rid)
+                    results.append((k, lab, info.batch_graphs))
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(e)
+
+        ts = [threading.Thread(target=client, args=(s,)) for s in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert not errors
+        assert len(results) == 4 * 3 * len(ids)
+        for k, lab, _ in results:
+            assert np.array_equal(lab, alone[k])
+        assert max(b for _, _, b in results) > 1  # requests were batched together
+        st = srv.stats()
+        assert st.requests_done == len(ids) + len(results)
+        assert st.batches < st.requests_done
+        assert st.p99_ms >= st.p50_ms > 0
+
+
+def test_device_graphs_match_single_solves(solver):
+    dgs = [solver.gen_rmat(V, ef, seed) for V, ef, seed in ((5000, 8.0, 1), (1 << 16, 16.0, 2), (30000, 4.0, 3))]
+    try:
+        with I.Server(0) as srv:
+            ids = [srv.add_graph(dg) for dg in dgs]
+            rids = [srv.submit(gid) for gid in ids for _ in range(2)]
+            labs = [srv.wait(r)[0] for r in rids]
+        for i, dg in enumerate(dgs):
+            solver.bind(dg)
+            solver.solve(I.default_params())
+            single = solver.labels()
+            assert np.array_equal(labs[2 * i], single) and np.array_equal(labs[2 * i + 1], single)
+    finally:
+        for dg in dgs:
+            dg.free()
+
+
+def test_budget_admission_and_removal():
+    small = O.rmat(8, 8, 1)
+    big = O.rmat(14, 16, 1)
+    with I.Server(0, hbm_budget=1 << 20) as srv:
+        gid = srv.add_graph(host(small))
+        with pytest.raises(I.IspanError) as e:
+            srv.add_graph(host(big))
+        assert e.value.code == -9
+        lab, _ = srv.wait(srv.submit(gid))
+        assert np.array_equal(lab, O.tarjan(small))
+        before = srv.stats().bytes_resident
+        srv.remove_graph(gid)
+        assert srv.stats().bytes_resident < before
+        with pytest.raises(I.IspanError) as e:
+            srv.submit(gid)
+        assert e.value.code == -11
+
+
+def test_remove_with_requests_in_flight(pool):
+    graphs, ref = pool
+    with I.Server(0) as srv:
+        gid = srv.add_graph(host(graphs[3]))
+        rids = [srv.submit(gid) for _ in range(5)]
+        srv.remove_graph(gid)  # memory goes once these finish
+        for rid in rids:
+            lab, info = srv.wait(rid)
+            assert np.array_equal(lab, ref[3])
+        assert srv.stats().graphs_resident == 0
+
+
+def test_bad_inputs():
+    with I.Server(0) as srv:
+        g = O.rmat(7, 4, 2)
+        adj = g.fw_adj.copy()
+        adj[0] = len(g.fw_beg) + 5
+        with pytest.raises(I.IspanError) as e:
+            srv.add_graph(I.HostGraph(g.fw_beg, adj, g.bw_beg, g.bw_adj))
+        assert e.value.code == -2
+        with pytest.raises(I.IspanError) as e:
+            srv.submit(12345)
+        assert e.value.code == -11
+        with pytest.raises(I.IspanError) as e:
+            srv.wait(777)
+        assert e.value.code == -11
