@@ -138,6 +138,90 @@ def batch_leg(S, I, n_graphs: int, seed: int, steps: int, warmup: int) -> dict:
     return out
 
 
+def served_leg(S, I, rates, n_req: int, seed: int) -> dict:
+    """Served mode: one resident R-MAT graph per scale 12..22 (EF16, the
+    same (a,b,c,d)), registered with a server under a 200 GiB HBM budget.
+    Requests pick a scale uniformly and arrive as a Poisson stream at each
+    rate; latency is the server's submit -> labels-on-host time.  Every
+    result is compared with a single solve of its graph.  Saturated: all
+    requests submitted at once, against one solve per launch of the same mix
+    (solve time only)."""
+    import queue
+    import threading
+    rng = np.random.default_rng(seed + 7)
+    scales = list(range(12, 23))
+    p = I.default_params()
+    dgs = [S.gen_rmat(1 << sc, 16.0, seed * 1000 + sc) for sc in scales]
+    refs, single_ms = [], []
+    try:
+        for dg in dgs:
+            S.bind(dg, I.default_params(validate=0))
+            S.solve(p)
+            t0 = time.perf_counter()
+            S.solve(p)
+            single_ms.append((time.perf_counter() - t0) * 1e3)
+            refs.append(S.labels())
+        srv = I.Server(0, hbm_budget=200 << 30)
+        ids = [srv.add_graph(dg) for dg in dgs]
+    finally:
+        for dg in dgs:
+            dg.free()
+    out = {"workload": "R-MAT graphs, scale uniform in 12..22 (one resident graph per scale), EF16, "
+                       "(0.57,0.19,0.19,0.05); Poisson arrivals; 200 GiB HBM budget",
+           "single_solve_ms": dict(zip([f"s{s}" for s in scales], single_ms)), "rates": {}}
+
+    def run(arrivals, picks):
+        srv.reset_stats()
+        q, bad, done = queue.Queue(), [0], []
+
+        def waiter():
+            for _ in range(len(picks)):
+                rid, k = q.get()
+                lab, info = srv.wait(rid)
+                if not np.array_equal(lab, refs[k]):
+                    bad[0] += 1
+                done.append(info.batch_graphs)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, k in enumerate(picks):
+            target = t0 + arrivals[i]
+            while True:
+                dt = target - time.perf_counter()
+                if dt <= 0:
+                    break
+                time.sleep(dt if dt > 2e-3 else 0)
+            q.put((srv.submit(ids[int(k)]), int(k)))
+        th.join()
+        elapsed = time.perf_counter() - t0
+        st = srv.stats()
+        return {"requests": len(picks), "achieved_graphs_per_s": len(picks) / elapsed, "p50_ms": st.p50_ms,
+                "p99_ms": st.p99_ms, "mean_ms": st.mean_ms, "max_ms": st.max_ms,
+                "mean_batch_graphs": st.mean_batch_graphs, "batches": st.batches, "labels_equal": bad[0] == 0}
+
+    try:
+        for rate in rates:
+            picks = rng.integers(0, len(ids), n_req)
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+            r = run(arrivals, picks)
+            r["offered_graphs_per_s"] = rate
+            out["rates"][str(rate)] = r
+            log(f"[served] {rate} req/s: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, "
+                f"{r['achieved_graphs_per_s']:.0f} graphs/s, batch {r['mean_batch_graphs']:.1f}, "
+                f"labels equal {r['labels_equal']}")
+        picks = rng.integers(0, len(ids), n_req)
+        sat = run(np.zeros(n_req), picks)
+        one = len(picks) / (sum(single_ms[int(k)] for k in picks) * 1e-3)
+        out["saturated"] = sat
+        out["one_per_launch_graphs_per_s"] = one
+        out["speedup_vs_one_per_launch"] = sat["achieved_graphs_per_s"] / one
+        log(f"[served] saturated {sat['achieved_graphs_per_s']:.0f} graphs/s vs one per launch {one:.0f}")
+    finally:
+        srv.close()
+    return out
+
+
 def cpu_leg(dg, V: int, E: int, gpu_labels: np.ndarray, threads: int, runs: int, timeout: float) -> dict:
     import oracle as O
     t0 = time.time()
@@ -192,6 +276,9 @@ def main(argv=None) -> int:
     ap.add_argument("--timing", type=int, default=2, help="library timing level inside the timed region")
     ap.add_argument("--batch-graphs", type=int, default=4096,
                     help="batched leg: number of graphs (0 skips it)")
+    ap.add_argument("--served-requests", type=int, default=1000,
+                    help="served leg: requests per arrival rate (0 skips it)")
+    ap.add_argument("--served-rates", default="100,500,1000,2000,5000", help="served leg: Poisson rates, req/s")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -299,10 +386,14 @@ def main(argv=None) -> int:
                 d = json.load(f)
             result["roofline"]["traffic"] = d.get("hbm_bytes_per_launch")
             result["roofline"]["traffic_source"] = os.path.relpath(pmc, ROOT)
-    if a.batch_graphs > 0:
+    if a.batch_graphs > 0 or a.served_requests > 0:
         dg.free()
         dg = None
+    if a.batch_graphs > 0:
         result["batched"] = batch_leg(S, I, a.batch_graphs, a.seed, max(1, a.steps // 2), 1)
+    if a.served_requests > 0:
+        rates = [int(x) for x in a.served_rates.split(",") if x]
+        result["served"] = served_leg(S, I, rates, a.served_requests, a.seed)
     if not a.no_cpu:
         if dg is None:
             dg, _ = build_graph(S, a.workload, a.seed)

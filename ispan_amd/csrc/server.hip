@@ -293,16 +293,7 @@ int add_graph_impl(ispan_server* S, int32_t V, int64_t E, const int32_t* const* 
             rc = ISPAN_ERR_HIP;
     }
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = ISPAN_ERR_HIP;
-    if (!rc && device) {
-        // device input: check the copy on the host side of the library
-        std::vector<int32_t> h[4];
-        for (int k = 0; k < 4 && !rc; ++k) {
-            h[k].resize(n[k] / 4);
-            if (n[k] && hipMemcpy(h[k].data(), g.d[k], n[k], hipMemcpyDeviceToHost) != hipSuccess) rc = ISPAN_ERR_HIP;
-        }
-        if (!rc && ispan_csr_check(V, E, h[0].data(), h[1].data(), h[2].data(), h[3].data()) != ISPAN_OK)
-            rc = ISPAN_ERR_CSR;
-    }
+    if (!rc && device) rc = ispan::check_csr_device(V, E, g.d[0], g.d[1], g.d[2], g.d[3], s);  // the copy, on the device
     std::lock_guard<std::mutex> lk(S->mu);
     if (rc) {
         free_graph_mem(g);

@@ -1104,6 +1104,27 @@ void add_stats(ispan_stats& t, const ispan_stats& st) {
 
 namespace ispan {
 
+int check_csr_device(int32_t V, int64_t E, const int32_t* fwb, const int32_t* fwa, const int32_t* bwb,
+                     const int32_t* bwa, hipStream_t s) {
+    Ctl* d = nullptr;
+    Ctl h{};
+    int rc = ISPAN_OK;
+    if (hipMalloc(&d, sizeof(Ctl)) != hipSuccess) return ISPAN_ERR_NOMEM;
+    int dev = 0, cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (launch_reset_ctl(d, s) != hipSuccess ||
+        (V > 0 && (launch_check_csr(fwb, fwa, V, E, d, cu * 8, s) != hipSuccess ||
+                   launch_check_csr(bwb, bwa, V, E, d, cu * 8, s) != hipSuccess)) ||
+        hipMemcpyAsync(&h, d, sizeof(Ctl), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        rc = ISPAN_ERR_HIP;
+    else if (h.bad_csr)
+        rc = ISPAN_ERR_CSR;
+    (void)hipFree(d);
+    return rc;
+}
+
 int64_t batch_workspace_bytes(int64_t V, int64_t E) {
     // union CSR pair + first arrays, and the per-vertex solve state
     return 8 * E + 16 * (V + 1) + 80 * (V + 1) + 2 * giant_hub_capacity(E) * (int64_t)sizeof(HubItem);
