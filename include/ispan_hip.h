@@ -191,20 +191,23 @@ const int32_t* ispan_batch_labels_device(const ispan_ctx* ctx);
 
 /* ---- serving --------------------------------------------------------------
  * A request loop over device-resident graphs with continuous batching; no
- * counterpart in the reference.  Graphs are registered once (copied to the
- * device, checked, admitted against an HBM budget that also covers the
- * working state of the largest batch).  A request names a graph.  A worker
- * thread takes every pending request that fits the batch limits and the
- * budget, solves them as one union (one cooperative launch) and starts over
- * with what arrived meanwhile; label copies back to the host overlap the next
- * batch's solve.  Each request's labels equal those of a single solve of its
- * graph.  All functions are thread-safe. */
+ * counterpart in the reference.  Registered graphs are appended once to an
+ * arena (one CSR pair in device memory holding every resident graph side by
+ * side); registration is checked and admitted against an HBM budget that
+ * covers the arena, the solve state sized for it and the label buffers.  A
+ * request names a graph.  A worker thread takes every pending request within
+ * the batch limits, solves the distinct graphs among them in one pass over
+ * the arena (nothing is copied per batch) and starts over with what arrived
+ * meanwhile.  Labels stay in device memory until the client takes them.
+ * Each request's labels equal those of a single solve of its graph.
+ * Latency is submit -> labels ready on the device.  All functions are
+ * thread-safe. */
 typedef struct ispan_server ispan_server;
 typedef struct {
-    int64_t hbm_budget;       /* device bytes for resident graphs + batch state (0: 200 GiB) */
-    int32_t max_batch_graphs; /* requests per batch (0: no limit)                          */
+    int64_t hbm_budget;         /* device bytes for the arena, solve state, labels (0: 200 GiB) */
+    int32_t max_batch_graphs;   /* distinct graphs per batch (0: no limit)                      */
     int32_t reserved0;
-    int64_t max_batch_edges;  /* edges per batch (0: 2^31 - 1)                             */
+    int64_t max_batch_vertices; /* vertices per batch (0: no limit)                             */
     int64_t reserved[5];
 } ispan_server_config;
 typedef struct {
@@ -215,30 +218,38 @@ typedef struct {
     int32_t status;                      /* ISPAN_OK or the batch's error          */
 } ispan_request_info;
 typedef struct {
-    int64_t requests_done, batches, graphs_resident, bytes_resident, bytes_budget;
-    double p50_ms, p99_ms, mean_ms, max_ms; /* latency (submit -> labels on the host) */
+    int64_t requests_done, batches, graphs_resident;
+    int64_t bytes_resident;             /* the live graphs' share of the arena     */
+    int64_t bytes_budget, bytes_device; /* budget; arena + solve state + labels    */
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency since the last reset        */
     double mean_batch_graphs;
-    int64_t reserved[6];
+    int64_t reserved[5];
 } ispan_server_stats;
 
 int ispan_server_create(ispan_server** out, int device, const ispan_server_config* cfg, const ispan_params* params);
 /* Finishes every submitted request, then stops. */
 int ispan_server_destroy(ispan_server* srv);
-/* Register a graph from host (checked in full) or device memory (copied, then
- * checked); ISPAN_ERR_BUDGET when it does not fit. */
+/* Register a graph from host or device memory (checked in full first);
+ * ISPAN_ERR_BUDGET when the arena cannot take it. */
 int ispan_server_add_graph(ispan_server* srv, int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj,
                            const int32_t* bw_beg, const int32_t* bw_adj, int64_t* graph_id);
 int ispan_server_add_graph_device(ispan_server* srv, int32_t V, int64_t E, const int32_t* d_fw_beg,
                                   const int32_t* d_fw_adj, const int32_t* d_bw_beg, const int32_t* d_bw_adj,
                                   int64_t* graph_id);
-/* Unregister; the memory is released once its pending requests finish. */
+/* Unregister (pending requests still finish).  Arena space is reused once
+ * every graph after it is gone too. */
 int ispan_server_remove_graph(ispan_server* srv, int64_t graph_id);
 /* Queue a solve of graph_id; returns at once. */
 int ispan_server_submit(ispan_server* srv, int64_t graph_id, int64_t* request_id);
-/* Wait for a request (timeout_ms < 0: no limit); labels: int32[V] or NULL.
- * Returns the request's status; the request is forgotten afterwards. */
+/* Wait for a request (timeout_ms < 0: no limit), copy its labels (int32[V])
+ * to host memory if labels is not NULL, and forget it.  Returns its status. */
 int ispan_server_wait(ispan_server* srv, int64_t request_id, int32_t* labels, ispan_request_info* info,
                       int64_t timeout_ms);
+/* Wait and leave the labels in device memory: *d_labels stays valid until
+ * ispan_server_release(request_id). */
+int ispan_server_wait_device(ispan_server* srv, int64_t request_id, const int32_t** d_labels, ispan_request_info* info,
+                             int64_t timeout_ms);
+int ispan_server_release(ispan_server* srv, int64_t request_id);
 int ispan_server_stats_get(ispan_server* srv, ispan_server_stats* out);
 int ispan_server_reset_stats(ispan_server* srv);
 const char* ispan_server_last_error(const ispan_server* srv);

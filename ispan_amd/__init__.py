@@ -40,7 +40,8 @@ EXPORTS = [
     "ispan_convert_edge_list", "ispan_batch_bind_device", "ispan_batch_groups", "ispan_batch_solve",
     "ispan_batch_labels_download", "ispan_batch_labels_device", "ispan_server_create", "ispan_server_destroy",
     "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit",
-    "ispan_server_wait", "ispan_server_stats_get", "ispan_server_reset_stats", "ispan_server_last_error",
+    "ispan_server_wait", "ispan_server_wait_device", "ispan_server_release", "ispan_server_stats_get",
+    "ispan_server_reset_stats", "ispan_server_last_error",
 ]
 
 
@@ -83,7 +84,7 @@ class Stats(C.Structure):
 class ServerConfig(C.Structure):
     """ispan_server_config (include/ispan_hip.h)."""
     _fields_ = [("hbm_budget", C.c_int64), ("max_batch_graphs", C.c_int32), ("reserved0", C.c_int32),
-                ("max_batch_edges", C.c_int64), ("reserved", C.c_int64 * 5)]
+                ("max_batch_vertices", C.c_int64), ("reserved", C.c_int64 * 5)]
 
 
 class RequestInfo(C.Structure):
@@ -94,9 +95,9 @@ class RequestInfo(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_int64), ("batches", C.c_int64), ("graphs_resident", C.c_int64),
-                ("bytes_resident", C.c_int64), ("bytes_budget", C.c_int64), ("p50_ms", C.c_double),
-                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("mean_batch_graphs", C.c_double), ("reserved", C.c_int64 * 6)]
+                ("bytes_resident", C.c_int64), ("bytes_budget", C.c_int64), ("bytes_device", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("mean_batch_graphs", C.c_double), ("reserved", C.c_int64 * 5)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -179,6 +180,8 @@ def lib():
         "ispan_server_remove_graph": (C.c_int, [_vp, C.c_int64]),
         "ispan_server_submit": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int64)]),
         "ispan_server_wait": (C.c_int, [_vp, C.c_int64, _vp, C.POINTER(RequestInfo), C.c_int64]),
+        "ispan_server_wait_device": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(RequestInfo), C.c_int64]),
+        "ispan_server_release": (C.c_int, [_vp, C.c_int64]),
         "ispan_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
         "ispan_server_reset_stats": (C.c_int, [_vp]),
         "ispan_server_last_error": (C.c_char_p, [_vp]),
@@ -436,10 +439,10 @@ class Server:
     """Continuous-batching SCC server (ispan_server_*): graphs resident on the
     device, requests batched as they arrive."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch_graphs: int = 0, max_batch_edges: int = 0,
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch_graphs: int = 0, max_batch_vertices: int = 0,
                  params: Params | None = None):
         cfg = ServerConfig(hbm_budget=hbm_budget, max_batch_graphs=max_batch_graphs,
-                           max_batch_edges=max_batch_edges)
+                           max_batch_vertices=max_batch_vertices)
         h = C.c_void_p()
         _check(lib().ispan_server_create(C.byref(h), device, C.byref(cfg), C.byref(params) if params else None))
         self._srv = h.value
@@ -484,6 +487,17 @@ class Server:
             self._rid_V[rid] = V
         self._err(rc)
         return (out[:V] if out is not None else None), info
+
+    def wait_device(self, rid: int, timeout_ms: int = -1):
+        """-> (device pointer of the labels, RequestInfo); release(rid) frees them."""
+        p = C.c_void_p()
+        info = RequestInfo()
+        self._err(lib().ispan_server_wait_device(self._srv, rid, C.byref(p), C.byref(info), timeout_ms))
+        return p.value or 0, info
+
+    def release(self, rid: int):
+        self._rid_V.pop(rid, None)
+        self._err(lib().ispan_server_release(self._srv, rid))
 
     def stats(self) -> ServerStats:
         st = ServerStats()

@@ -132,6 +132,7 @@ struct GiantArgs {
     // their own id as label, to be replaced per graph afterwards
     const int32_t* roots;
     int32_t n_roots;          // 0: single graph, pivot chosen in the kernel
+    const uint64_t* sel;      // optional: only these vertices take part (the rest is never active)
     int32_t switch_rule;      // top-down -> bottom-up estimate of the frontier's work: 0 found x average
                               // degree (the reference's), 1 the frontier's exact out-degree sum
 };
@@ -148,14 +149,28 @@ hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, co
                               const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,
                               int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
                               int32_t* ubwa, int32_t* bad, hipStream_t s);
-hipError_t launch_batch_union_ptrs(const int32_t* const* fwb, const int32_t* const* fwa, const int32_t* const* bwb,
-                                   const int32_t* const* bwa, const int64_t* d_v_off, const int64_t* d_e_off,
-                                   int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
-                                   int32_t* ubwa, int32_t* bad, hipStream_t s);
-hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const int64_t* goff, int32_t ng, int32_t wrap32,
-                              int32_t* roots, hipStream_t s);
-hipError_t launch_batch_finish(const int32_t* lab, const uint64_t* bvis, const int64_t* goff, const int32_t* roots,
-                               int32_t ng, int64_t Vu, int32_t* gmin, int32_t* out, hipStream_t s);
+// Graph ranges of one batch solve (device arrays, see batch.hip): graph g
+// owns vertices [vb[g], ve[g]) of the CSR being solved; its local labels go
+// to out[g]; cstart is the prefix of its kChunk-vertex chunk counts.
+struct BatchRanges {
+    int32_t n = 0;
+    int64_t nchunks = 0;
+    int64_t* vb = nullptr;
+    int64_t* ve = nullptr;
+    int64_t* cstart = nullptr;
+    int32_t** out = nullptr;
+    int64_t* cp = nullptr;  // per-chunk pivot candidates
+    int32_t* ci = nullptr;
+    int32_t* roots = nullptr;
+    int32_t* gmin = nullptr;
+};
+constexpr int64_t kBatchChunk = 1 << 16;  // vertices per chunk (kChunk in batch.hip)
+hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const BatchRanges& r, int32_t wrap32,
+                              hipStream_t s);
+hipError_t launch_batch_finish(const int32_t* lab, const uint64_t* bvis, const BatchRanges& r, hipStream_t s);
+hipError_t launch_batch_select(const BatchRanges& r, uint64_t* sel, int64_t nwords, hipStream_t s);
+hipError_t launch_arena_append(const int32_t* beg, const int32_t* adj, int32_t V, int64_t E, int64_t vb, int64_t eb,
+                               int32_t* abeg, int32_t* aadj, hipStream_t s);
 hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s);
 
 // Grid sizes for grid-stride kernels.

@@ -669,8 +669,9 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     int32_t besti = INT32_MAX;
     for (int64_t w = gwave; w < a.nwords; w += total_waves) {
         const int64_t v = w * 64 + c.lane;
+        const uint64_t sw = a.sel ? a.sel[w] : ~0ull;  // vertices outside the selection never become active
         bool s = false;
-        if (v < a.V) {
+        if (v < a.V && ((sw >> c.lane) & 1ull)) {
             const int32_t od = a.fwb[v + 1] - a.fwb[v];
             const int32_t id = a.bwb[v + 1] - a.bwb[v];
             s = od > 0 && id > 0;

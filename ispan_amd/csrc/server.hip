@@ -3,16 +3,20 @@
 // src/scc_main.cpp:7-57); each request's labels are exactly those of a single
 // solve of its graph.
 //
-// * Graphs are registered once and stay in device memory until removed.
-//   Registration is admitted against an HBM budget that also covers the
-//   largest batch's working state.
+// * Registered graphs are appended once to an arena: one CSR pair in device
+//   memory holding every resident graph side by side with global ids, with
+//   its first-neighbour arrays.  Registration is admitted against an HBM
+//   budget that covers the arena, the solve state sized for it and the
+//   outstanding label buffers.
 // * A request names a graph.  The worker thread takes every pending request
-//   that fits the batch limits and the budget, solves them as one union (one
-//   cooperative launch), and starts again with whatever arrived meanwhile:
-//   requests that arrive while a batch runs join the next one.
-// * Labels come back through two device/pinned slots: the copy of batch k's
-//   labels (second stream) overlaps batch k+1's solve; a completion thread
-//   hands them to the waiting requests.
+//   within the batch limits, solves the distinct graphs among them in one pass
+//   over the arena (a selection bitmap keeps every other vertex inactive, so
+//   nothing is copied per batch), and starts again with whatever arrived
+//   meanwhile.  Requests that arrive while a batch runs join the next one.
+// * Labels stay in device memory (a pool of batch buffers) until the client
+//   takes them: copied to the host by wait(), or read in place through
+//   wait_device() and released.  Latency is submit -> labels ready on the
+//   device.
 #include <string.h>
 
 #include <hip/hip_runtime.h>
@@ -21,6 +25,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -28,6 +33,7 @@
 #include <vector>
 
 #include "../../include/ispan_hip.h"
+#include "common.h"
 #include "internal.h"
 
 namespace {
@@ -38,12 +44,18 @@ double now_ms() {
 }
 
 struct Graph {
+    int64_t vb = 0;  // first arena vertex
+    int64_t eb = 0;  // first arena edge
     int32_t V = 0;
     int64_t E = 0;
-    int32_t* d[4] = {nullptr, nullptr, nullptr, nullptr};
-    int64_t bytes = 0;
     int pending = 0;  // requests not yet finished
     bool removed = false;
+};
+
+struct LabelBuf {
+    int32_t* d = nullptr;
+    int64_t cap = 0;  // int32 entries
+    int refs = 0;     // requests still holding part of it
 };
 
 struct Request {
@@ -54,16 +66,16 @@ struct Request {
     int status = ISPAN_OK;
     int64_t batch = -1;
     int32_t batch_graphs = 0;
-    std::vector<int32_t> labels;
+    int buf = -1;            // label buffer
+    int32_t* d_labels = nullptr;
 };
 
-// One batch in flight between the worker and the completion thread.
+// One batch between the worker and the completion thread.
 struct Flight {
     std::vector<int64_t> reqs;
-    std::vector<int64_t> v_off;
-    int slot = 0;
+    int buf = -1;
     int status = ISPAN_OK;
-    hipEvent_t copied = nullptr;
+    hipEvent_t done = nullptr;
     int64_t id = 0;
 };
 
@@ -75,24 +87,26 @@ struct ispan_server {
     ispan_params params{};
     int64_t budget = 0;
     int32_t max_graphs = 0;
-    int64_t max_edges = 0;
+    int64_t max_vertices = 0;
     hipStream_t compute_stream = nullptr;  // the solver context's stream
-    hipStream_t copy_stream = nullptr;
+    hipStream_t copy_stream = nullptr;     // registration
 
     std::mutex mu;
-    std::condition_variable cv_work, cv_done, cv_flight, cv_slot;
-    std::unordered_map<int64_t, Graph> graphs;
+    std::condition_variable cv_work, cv_done, cv_flight, cv_mem, cv_arena;
+    // arena (grown only while no batch is in flight)
+    int32_t* csr[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t* first[2] = {nullptr, nullptr};
+    int64_t capV = 0, capE = 0, usedV = 0, usedE = 0;
+    int inflight = 0;     // batches whose kernels may still read the arena
+    bool growing = false;
+    std::map<int64_t, Graph> graphs;  // ordered by id = arena order
     std::unordered_map<int64_t, Request> reqs;
     std::deque<int64_t> pending;
     std::deque<Flight> flights;
+    std::vector<LabelBuf> bufs;
     int64_t next_graph = 1, next_req = 1, next_batch = 0;
-    int64_t bytes_graphs = 0, bytes_ws = 0;
-    int64_t ws_need = 0;  // working-state bytes the largest admitted graph needs alone
+    int64_t bytes_live = 0;  // live graphs' arena share
     bool stop = false;
-    bool slot_busy[2] = {false, false};
-    int32_t* d_labels[2] = {nullptr, nullptr};
-    int32_t* h_labels[2] = {nullptr, nullptr};  // pinned
-    int64_t cap_labels[2] = {0, 0};
     std::vector<double> latencies;
     int64_t batches = 0, batch_graphs_total = 0;
     std::string err;
@@ -101,101 +115,170 @@ struct ispan_server {
 
 namespace {
 
-void free_graph_mem(Graph& g) {
-    for (auto& p : g.d)
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
+int64_t arena_bytes(int64_t V, int64_t E) { return 2 * (V + 1) * 4 + 2 * std::max<int64_t>(E, 1) * 4 + 2 * V * 4; }
+
+int64_t label_bytes(const ispan_server* S) {
+    int64_t b = 0;
+    for (const auto& x : S->bufs) b += x.cap * 4;
+    return b;
 }
 
-// Fills a batch from the front of the queue: FIFO, bounded by the graph and
-// edge limits, the int32 union and the working-state budget.
+// Device bytes in use or reserved: arena capacity, solve state for it, label buffers.
+int64_t bytes_used(const ispan_server* S) {
+    return arena_bytes(S->capV, S->capE) + ispan::solve_state_bytes(S->capV, S->capE) + label_bytes(S);
+}
+
+// A label buffer of at least n entries: a free one, else a new one within the
+// budget (-1: none fits right now).  Called with mu held.
+int acquire_buf(ispan_server* S, int64_t n) {
+    int best = -1;
+    for (int i = 0; i < (int)S->bufs.size(); ++i)
+        if (S->bufs[i].refs == 0 && S->bufs[i].cap >= n && (best < 0 || S->bufs[i].cap < S->bufs[best].cap)) best = i;
+    if (best >= 0) return best;
+    int64_t cap = std::max<int64_t>(n, 1 << 20);  // small batches share a reusable size
+    if (bytes_used(S) + cap * 4 > S->budget) cap = n;
+    if (bytes_used(S) + cap * 4 > S->budget) {
+        // free idle buffers that are too small, then retry once
+        for (auto& b : S->bufs)
+            if (b.refs == 0 && b.d) {
+                (void)hipFree(b.d);
+                b.d = nullptr;
+                b.cap = 0;
+            }
+        if (bytes_used(S) + cap * 4 > S->budget) return -1;
+    }
+    LabelBuf b;
+    if (hipMalloc(&b.d, (size_t)cap * 4) != hipSuccess) return -1;
+    b.cap = cap;
+    for (int i = 0; i < (int)S->bufs.size(); ++i)
+        if (!S->bufs[i].d) {
+            S->bufs[i] = b;
+            return i;
+        }
+    S->bufs.push_back(b);
+    return (int)S->bufs.size() - 1;
+}
+
+void drop_ref(ispan_server* S, Request& r) {
+    if (r.buf >= 0 && --S->bufs[r.buf].refs == 0) S->cv_mem.notify_all();
+    r.buf = -1;
+    r.d_labels = nullptr;
+}
+
+// Arena space of removed graphs at its end is reused.  Called with mu held.
+void trim_arena(ispan_server* S) {
+    while (!S->graphs.empty()) {
+        auto it = std::prev(S->graphs.end());
+        if (!it->second.removed || it->second.pending) break;
+        S->usedV = it->second.vb;
+        S->usedE = it->second.eb;
+        S->graphs.erase(it);
+    }
+    if (S->graphs.empty()) S->usedV = S->usedE = 0;
+}
+
+// Fills a batch from the front of the queue, FIFO, within the graph and
+// vertex limits and the label memory the budget leaves.
 void take_batch(ispan_server* S, std::vector<int64_t>& out) {
-    int64_t V = 0, E = 0;
+    std::unordered_map<int64_t, int> seen;
+    int64_t V = 0, busy = 0;
+    for (const auto& b : S->bufs) busy += b.refs ? b.cap * 4 : 0;
+    const int64_t room = S->budget - arena_bytes(S->capV, S->capE) - ispan::solve_state_bytes(S->capV, S->capE) - busy;
     while (!S->pending.empty()) {
         const int64_t rid = S->pending.front();
         const Request& r = S->reqs[rid];
-        const Graph& g = S->graphs[r.graph];
-        const int64_t V2 = V + g.V, E2 = E + g.E;
-        const bool first = out.empty();
-        if (!first) {
-            if (S->max_graphs > 0 && (int32_t)out.size() >= S->max_graphs) break;
-            if (E2 > S->max_edges || V2 > INT32_MAX - 64) break;
-            if (S->bytes_graphs + ispan::batch_workspace_bytes(V2, E2) > S->budget) break;
+        const bool is_new = !seen.count(r.graph);
+        const int64_t V2 = V + (is_new ? r.V : 0);
+        if (!out.empty() && is_new) {
+            if (S->max_graphs > 0 && (int32_t)seen.size() >= S->max_graphs) break;
+            if (S->max_vertices > 0 && V2 > S->max_vertices) break;
+            if (4 * V2 > room) break;
         }
+        if (is_new) seen[r.graph] = 1;
         out.push_back(rid);
         V = V2;
-        E = E2;
         S->pending.pop_front();
     }
 }
 
-int ensure_slot(ispan_server* S, int slot, int64_t n) {
-    if (n <= S->cap_labels[slot]) return ISPAN_OK;
-    if (S->d_labels[slot]) (void)hipFree(S->d_labels[slot]);
-    if (S->h_labels[slot]) (void)hipHostFree(S->h_labels[slot]);
-    S->d_labels[slot] = nullptr;
-    S->h_labels[slot] = nullptr;
-    S->cap_labels[slot] = 0;
-    const int64_t cap = std::max<int64_t>(n, 1 << 20);
-    if (hipMalloc(&S->d_labels[slot], (size_t)cap * 4) != hipSuccess) return ISPAN_ERR_NOMEM;
-    if (hipHostMalloc(&S->h_labels[slot], (size_t)cap * 4, hipHostMallocDefault) != hipSuccess) return ISPAN_ERR_NOMEM;
-    S->cap_labels[slot] = cap;
-    return ISPAN_OK;
-}
-
 void worker_loop(ispan_server* S) {
     (void)hipSetDevice(S->device);
-    int slot = 0;
     for (;;) {
         std::vector<int64_t> batch;
-        std::vector<int64_t> v_off(1, 0), e_off(1, 0);
-        std::vector<const int32_t*> t[4];
+        std::vector<int64_t> vb, ve;
+        std::vector<int32_t*> out;
+        std::unordered_map<int64_t, int64_t> offset;  // graph -> offset in the label buffer
+        Flight f;
+        ispan::ArenaView av{};
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [S] { return S->stop || !S->pending.empty(); });
             if (S->stop && S->pending.empty()) break;
             take_batch(S, batch);
+            int64_t n = 0;
+            for (int64_t rid : batch) {
+                const Request& r = S->reqs[rid];
+                if (!offset.count(r.graph)) {
+                    offset[r.graph] = n;
+                    n += r.V;
+                }
+            }
+            // a label buffer for the batch, once memory allows; if it cannot
+            // fit even with every buffer released, the batch fails
+            int b = -1;
+            bool never = false;
+            S->cv_mem.wait(lk, [&] {
+                if ((b = acquire_buf(S, std::max<int64_t>(n, 1))) >= 0) return true;
+                bool busy = false;
+                for (const auto& x : S->bufs) busy |= x.refs > 0;
+                never = !busy;
+                return never || S->stop;
+            });
+            if (b < 0) {
+                const double t = now_ms();
+                for (int64_t rid : batch) {
+                    Request& r = S->reqs[rid];
+                    r.status = ISPAN_ERR_BUDGET;
+                    r.t_start = r.t_done = t;
+                    r.done = true;
+                    auto it = S->graphs.find(r.graph);
+                    if (it != S->graphs.end() && --it->second.pending == 0 && it->second.removed) trim_arena(S);
+                }
+                S->err = "the labels of a batch do not fit the HBM budget";
+                S->cv_done.notify_all();
+                continue;
+            }
+            S->cv_arena.wait(lk, [S] { return !S->growing; });
+            S->inflight += 1;
+            f.buf = b;
+            LabelBuf& lb = S->bufs[b];
+            lb.refs = (int)batch.size();
             const double t0 = now_ms();
             for (int64_t rid : batch) {
                 Request& r = S->reqs[rid];
-                const Graph& g = S->graphs[r.graph];
                 r.t_start = t0;
-                v_off.push_back(v_off.back() + g.V);
-                e_off.push_back(e_off.back() + g.E);
-                for (int k = 0; k < 4; ++k) t[k].push_back(g.d[k]);
+                r.buf = b;
+                r.d_labels = lb.d + offset[r.graph];
             }
-            // a label slot is reused only after its previous copy has been consumed
-            S->cv_slot.wait(lk, [S, slot] { return !S->slot_busy[slot]; });
-            S->slot_busy[slot] = true;
+            for (const auto& kv : offset) {
+                const Graph& g = S->graphs[kv.first];
+                if (g.V == 0) continue;
+                vb.push_back(g.vb);
+                ve.push_back(g.vb + g.V);
+                out.push_back(lb.d + kv.second);
+            }
+            av = ispan::ArenaView{S->csr[0], S->csr[1], S->csr[2], S->csr[3], S->first[0], S->first[1],
+                                  (int32_t)S->usedV, S->usedE};
         }
-        Flight f;
         f.reqs = batch;
-        f.v_off = v_off;
-        f.slot = slot;
-        int rc = ensure_slot(S, slot, v_off.back());
-        ispan_stats st{};
-        if (!rc)
-            rc = ispan::batch_solve_tables(S->ctx, (int32_t)batch.size(), v_off.data(), e_off.data(), t[0].data(),
-                                           t[1].data(), t[2].data(), t[3].data(), &S->params, S->d_labels[slot], &st);
-        if (!rc) {
-            // labels to pinned memory on the copy stream, after the solve's
-            // last kernel on the compute stream; the next batch's solve runs
-            // meanwhile
-            hipEvent_t solved = nullptr;
-            if (hipEventCreateWithFlags(&solved, hipEventDisableTiming) != hipSuccess ||
-                hipEventRecord(solved, S->compute_stream) != hipSuccess ||
-                hipStreamWaitEvent(S->copy_stream, solved, 0) != hipSuccess ||
-                hipEventCreateWithFlags(&f.copied, hipEventDisableTiming) != hipSuccess)
-                rc = ISPAN_ERR_HIP;
-            if (!rc && v_off.back() > 0 &&
-                hipMemcpyAsync(S->h_labels[slot], S->d_labels[slot], (size_t)v_off.back() * 4,
-                               hipMemcpyDeviceToHost, S->copy_stream) != hipSuccess)
-                rc = ISPAN_ERR_HIP;
-            if (!rc && hipEventRecord(f.copied, S->copy_stream) != hipSuccess) rc = ISPAN_ERR_HIP;
-            if (solved) (void)hipEventDestroy(solved);
+        int rc = ISPAN_OK;
+        if (!vb.empty()) {
+            ispan_stats st{};
+            rc = ispan::solve_arena(S->ctx, av, (int32_t)vb.size(), vb.data(), ve.data(), out.data(), &S->params, &st);
         }
+        if (!rc && (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess ||
+                    hipEventRecord(f.done, S->compute_stream) != hipSuccess))
+            rc = ISPAN_ERR_HIP;
         f.status = rc;
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -204,7 +287,6 @@ void worker_loop(ispan_server* S) {
             S->flights.push_back(std::move(f));
         }
         S->cv_flight.notify_one();
-        slot ^= 1;
     }
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -226,82 +308,128 @@ void completer_loop(ispan_server* S) {
             S->flights.pop_front();
         }
         if (f.id < 0) break;
-        if (f.copied) {
-            if (hipEventSynchronize(f.copied) != hipSuccess && f.status == ISPAN_OK) f.status = ISPAN_ERR_HIP;
-            (void)hipEventDestroy(f.copied);
+        if (f.done) {
+            if (hipEventSynchronize(f.done) != hipSuccess && f.status == ISPAN_OK) f.status = ISPAN_ERR_HIP;
+            (void)hipEventDestroy(f.done);
+        } else if (f.status == ISPAN_OK) {
+            (void)hipStreamSynchronize(S->compute_stream);
         }
         const double t = now_ms();
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            for (size_t i = 0; i < f.reqs.size(); ++i) {
-                Request& r = S->reqs[f.reqs[i]];
-                if (f.status == ISPAN_OK) {
-                    r.labels.assign(S->h_labels[f.slot] + f.v_off[i], S->h_labels[f.slot] + f.v_off[i + 1]);
-                }
+            for (int64_t rid : f.reqs) {
+                Request& r = S->reqs[rid];
                 r.status = f.status;
                 r.t_done = t;
                 r.batch = f.id;
                 r.batch_graphs = (int32_t)f.reqs.size();
                 r.done = true;
+                if (f.status != ISPAN_OK) drop_ref(S, r);
                 S->latencies.push_back(t - r.t_submit);
                 auto it = S->graphs.find(r.graph);
-                if (it != S->graphs.end() && --it->second.pending == 0 && it->second.removed) {
-                    S->bytes_graphs -= it->second.bytes;
-                    free_graph_mem(it->second);
-                    S->graphs.erase(it);
-                }
+                if (it != S->graphs.end() && --it->second.pending == 0 && it->second.removed) trim_arena(S);
             }
+            S->inflight -= 1;
             S->batches += 1;
             S->batch_graphs_total += (int64_t)f.reqs.size();
-            S->slot_busy[f.slot] = false;
         }
-        S->cv_slot.notify_all();
+        S->cv_arena.notify_all();
         S->cv_done.notify_all();
     }
 }
 
-int add_graph_impl(ispan_server* S, int32_t V, int64_t E, const int32_t* const* src, bool device, int64_t* id) {
-    if (!S || V < 0 || E < 0 || E > INT32_MAX || !id || !src[0] || !src[2] || (E > 0 && (!src[1] || !src[3])))
-        return ISPAN_ERR_ARG;
-    if (!device) {
-        if (src[0][0] != 0 || src[2][0] != 0 || (int64_t)src[0][V] != E || (int64_t)src[2][V] != E)
-            return ISPAN_ERR_CSR;
-        // the host arrays are checked in full before they are admitted
-        if (ispan_csr_check(V, E, src[0], src[1], src[2], src[3]) != ISPAN_OK) return ISPAN_ERR_CSR;
+// Grows the arena to hold V more vertices and E more edges (called with the
+// lock held and no batch in flight; the lock is dropped during the copy).
+int grow_arena(ispan_server* S, std::unique_lock<std::mutex>& lk, int64_t needV, int64_t needE) {
+    int64_t nV = std::max<int64_t>(needV, std::min<int64_t>(2 * S->capV, INT32_MAX - 64));
+    int64_t nE = std::max<int64_t>(needE, std::min<int64_t>(2 * S->capE, INT32_MAX));
+    // doubling only while the budget allows it
+    // room is kept for the labels of one batch as large as the arena
+    auto fits = [&](int64_t v, int64_t e) {
+        return arena_bytes(v, e) + ispan::solve_state_bytes(v, e) + 4 * (v + 1) <= S->budget;
+    };
+    if (!fits(nV, nE)) {
+        nV = needV;
+        nE = needE;
+        if (!fits(nV, nE)) return ISPAN_ERR_BUDGET;
     }
-    const int64_t bytes = 2 * ((int64_t)V + 1) * 4 + 2 * std::max<int64_t>(E, 1) * 4;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        const int64_t need = std::max(S->ws_need, ispan::batch_workspace_bytes(V, E));
-        if (S->bytes_graphs + bytes + need > S->budget) return ISPAN_ERR_BUDGET;
-        S->bytes_graphs += bytes;  // reserved while copying
-        S->ws_need = need;
-    }
-    Graph g;
-    g.V = V;
-    g.E = E;
-    g.bytes = bytes;
-    const size_t sz[4] = {((size_t)V + 1) * 4, (size_t)std::max<int64_t>(E, 1) * 4, ((size_t)V + 1) * 4,
-                          (size_t)std::max<int64_t>(E, 1) * 4};
-    const size_t n[4] = {((size_t)V + 1) * 4, (size_t)E * 4, ((size_t)V + 1) * 4, (size_t)E * 4};
+    S->growing = true;
+    lk.unlock();
+    int32_t* n[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const size_t sz[6] = {((size_t)nV + 1) * 4, (size_t)std::max<int64_t>(nE, 1) * 4, ((size_t)nV + 1) * 4,
+                          (size_t)std::max<int64_t>(nE, 1) * 4, (size_t)std::max<int64_t>(nV, 1) * 4,
+                          (size_t)std::max<int64_t>(nV, 1) * 4};
+    const size_t used[6] = {((size_t)S->usedV + 1) * 4, (size_t)S->usedE * 4, ((size_t)S->usedV + 1) * 4,
+                            (size_t)S->usedE * 4, (size_t)S->usedV * 4, (size_t)S->usedV * 4};
+    int32_t* old[6] = {S->csr[0], S->csr[1], S->csr[2], S->csr[3], S->first[0], S->first[1]};
     int rc = ISPAN_OK;
-    hipStream_t s = S->copy_stream;
-    for (int k = 0; k < 4 && !rc; ++k) {
-        if (hipMalloc(&g.d[k], sz[k]) != hipSuccess) rc = ISPAN_ERR_NOMEM;
-        else if (n[k] && hipMemcpyAsync(g.d[k], src[k], n[k], device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                        s) != hipSuccess)
+    for (int k = 0; k < 6 && !rc; ++k) {
+        if (hipMalloc(&n[k], sz[k]) != hipSuccess) rc = ISPAN_ERR_NOMEM;
+        else if (old[k] && used[k] && hipMemcpyAsync(n[k], old[k], used[k], hipMemcpyDeviceToDevice, S->copy_stream))
             rc = ISPAN_ERR_HIP;
     }
-    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = ISPAN_ERR_HIP;
-    if (!rc && device) rc = ispan::check_csr_device(V, E, g.d[0], g.d[1], g.d[2], g.d[3], s);  // the copy, on the device
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (rc) {
-        free_graph_mem(g);
-        S->bytes_graphs -= bytes;
-        return rc;
+    if (!rc && hipStreamSynchronize(S->copy_stream) != hipSuccess) rc = ISPAN_ERR_HIP;
+    if (!rc && !old[0]) {  // empty arena: offset 0 at vertex 0
+        const int32_t zero = 0;
+        if (hipMemcpy(n[0], &zero, 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(n[2], &zero, 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = ISPAN_ERR_HIP;
     }
+    lk.lock();
+    if (rc) {
+        for (auto* p : n)
+            if (p) (void)hipFree(p);
+    } else {
+        for (auto* p : old)
+            if (p) (void)hipFree(p);
+        for (int k = 0; k < 4; ++k) S->csr[k] = n[k];
+        S->first[0] = n[4];
+        S->first[1] = n[5];
+        S->capV = nV;
+        S->capE = nE;
+    }
+    S->growing = false;
+    S->cv_arena.notify_all();
+    return rc;
+}
+
+// Registers a graph whose CSR pair is in device memory (checked here).
+int register_device(ispan_server* S, int32_t V, int64_t E, const int32_t* const* d, int64_t* id) {
+    hipStream_t s = S->copy_stream;
+    int rc = ispan::check_csr_device(V, E, d[0], d[1], d[2], d[3], s);
+    if (rc) return rc;
+    std::unique_lock<std::mutex> lk(S->mu);
+    // growth and appends wait until no batch reads the arena
+    S->cv_arena.wait(lk, [S] { return S->inflight == 0 && !S->growing; });
+    const int64_t vb = S->usedV, eb = S->usedE;
+    if (vb + V > INT32_MAX - 64 || eb + E > INT32_MAX) return ISPAN_ERR_BUDGET;
+    if (vb + V > S->capV || eb + E > S->capE) {
+        rc = grow_arena(S, lk, vb + V, eb + E);
+        if (rc) return rc;
+        S->cv_arena.wait(lk, [S] { return S->inflight == 0 && !S->growing; });
+    }
+    S->growing = true;  // keeps batches out while the graph is appended
+    lk.unlock();
+    if (ispan::launch_arena_append(d[0], d[1], V, E, vb, eb, S->csr[0], S->csr[1], s) != hipSuccess ||
+        ispan::launch_arena_append(d[2], d[3], V, E, vb, eb, S->csr[2], S->csr[3], s) != hipSuccess ||
+        (V > 0 && (ispan::launch_first(S->csr[0] + vb, S->csr[1], V, S->first[0] + vb, 1024, s) != hipSuccess ||
+                   ispan::launch_first(S->csr[2] + vb, S->csr[3], V, S->first[1] + vb, 1024, s) != hipSuccess)) ||
+        hipStreamSynchronize(s) != hipSuccess)
+        rc = ISPAN_ERR_HIP;
+    lk.lock();
+    S->growing = false;
+    S->cv_arena.notify_all();
+    if (rc) return rc;
+    Graph g;
+    g.vb = vb;
+    g.eb = eb;
+    g.V = V;
+    g.E = E;
+    S->usedV = vb + V;
+    S->usedE = eb + E;
+    S->bytes_live += arena_bytes(V, E);
     *id = S->next_graph++;
-    S->graphs.emplace(*id, std::move(g));
+    S->graphs.emplace(*id, g);
     return ISPAN_OK;
 }
 
@@ -312,6 +440,26 @@ double percentile(std::vector<double> v, double q) {
     const size_t i = (size_t)pos;
     const double f = pos - (double)i;
     return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+int wait_done(ispan_server* S, std::unique_lock<std::mutex>& lk, int64_t rid, int64_t timeout_ms) {
+    auto it = S->reqs.find(rid);
+    if (it == S->reqs.end()) return ISPAN_ERR_NOTFOUND;
+    auto ready = [S, rid] { return S->reqs[rid].done; };
+    if (timeout_ms < 0) S->cv_done.wait(lk, ready);
+    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return ISPAN_ERR_TIMEOUT;
+    return ISPAN_OK;
+}
+
+void fill_info(const Request& r, ispan_request_info* info) {
+    if (!info) return;
+    info->submit_ms = r.t_submit;
+    info->start_ms = r.t_start;
+    info->done_ms = r.t_done;
+    info->latency_ms = r.t_done - r.t_submit;
+    info->batch = r.batch;
+    info->batch_graphs = r.batch_graphs;
+    info->status = r.status;
 }
 
 }  // namespace
@@ -327,7 +475,7 @@ int ispan_server_create(ispan_server** out, int device, const ispan_server_confi
     if (params) S->params = *params;
     S->budget = cfg && cfg->hbm_budget > 0 ? cfg->hbm_budget : (int64_t)200 << 30;
     S->max_graphs = cfg ? std::max(cfg->max_batch_graphs, 0) : 0;
-    S->max_edges = cfg && cfg->max_batch_edges > 0 ? std::min<int64_t>(cfg->max_batch_edges, INT32_MAX) : INT32_MAX;
+    S->max_vertices = cfg && cfg->max_batch_vertices > 0 ? cfg->max_batch_vertices : 0;
     int rc = ispan_ctx_create(&S->ctx, device);
     if (!rc && (hipStreamCreateWithFlags(&S->compute_stream, hipStreamNonBlocking) != hipSuccess ||
                 hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess))
@@ -353,15 +501,17 @@ int ispan_server_destroy(ispan_server* S) {
         S->stop = true;
     }
     S->cv_work.notify_all();
+    S->cv_mem.notify_all();
     if (S->worker.joinable()) S->worker.join();
     if (S->completer.joinable()) S->completer.join();
     (void)hipSetDevice(S->device);
-    for (auto& kv : S->graphs) free_graph_mem(kv.second);
-    for (int k = 0; k < 2; ++k) {
-        if (S->d_labels[k]) (void)hipFree(S->d_labels[k]);
-        if (S->h_labels[k]) (void)hipHostFree(S->h_labels[k]);
-    }
-    ispan_ctx_destroy(S->ctx);
+    ispan_ctx_destroy(S->ctx);  // waits for its stream
+    for (auto* p : S->csr)
+        if (p) (void)hipFree(p);
+    for (auto* p : S->first)
+        if (p) (void)hipFree(p);
+    for (auto& b : S->bufs)
+        if (b.d) (void)hipFree(b.d);
     if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
     if (S->compute_stream) (void)hipStreamDestroy(S->compute_stream);
     delete S;
@@ -370,19 +520,37 @@ int ispan_server_destroy(ispan_server* S) {
 
 int ispan_server_add_graph(ispan_server* S, int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj,
                            const int32_t* bw_beg, const int32_t* bw_adj, int64_t* graph_id) {
-    const int32_t* src[4] = {fw_beg, fw_adj, bw_beg, bw_adj};
-    if (!S) return ISPAN_ERR_ARG;
+    if (!S || V < 0 || E < 0 || E > INT32_MAX || !graph_id || !fw_beg || !bw_beg || (E > 0 && (!fw_adj || !bw_adj)))
+        return ISPAN_ERR_ARG;
+    if (ispan_csr_check(V, E, fw_beg, fw_adj, bw_beg, bw_adj) != ISPAN_OK) return ISPAN_ERR_CSR;
     (void)hipSetDevice(S->device);
-    return add_graph_impl(S, V, E, src, false, graph_id);
+    // staged through temporary device arrays
+    int32_t* d[4] = {nullptr, nullptr, nullptr, nullptr};
+    const int32_t* h[4] = {fw_beg, fw_adj, bw_beg, bw_adj};
+    const size_t n[4] = {((size_t)V + 1) * 4, (size_t)E * 4, ((size_t)V + 1) * 4, (size_t)E * 4};
+    int rc = ISPAN_OK;
+    for (int k = 0; k < 4 && !rc; ++k) {
+        if (hipMalloc(&d[k], std::max<size_t>(n[k], 4)) != hipSuccess) rc = ISPAN_ERR_NOMEM;
+        else if (n[k] && hipMemcpy(d[k], h[k], n[k], hipMemcpyHostToDevice) != hipSuccess) rc = ISPAN_ERR_HIP;
+    }
+    if (!rc) {
+        const int32_t* dc[4] = {d[0], d[1], d[2], d[3]};
+        rc = register_device(S, V, E, dc, graph_id);
+    }
+    for (auto* p : d)
+        if (p) (void)hipFree(p);
+    return rc;
 }
 
 int ispan_server_add_graph_device(ispan_server* S, int32_t V, int64_t E, const int32_t* d_fw_beg,
                                   const int32_t* d_fw_adj, const int32_t* d_bw_beg, const int32_t* d_bw_adj,
                                   int64_t* graph_id) {
-    const int32_t* src[4] = {d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj};
-    if (!S) return ISPAN_ERR_ARG;
+    if (!S || V < 0 || E < 0 || E > INT32_MAX || !graph_id || !d_fw_beg || !d_bw_beg ||
+        (E > 0 && (!d_fw_adj || !d_bw_adj)))
+        return ISPAN_ERR_ARG;
     (void)hipSetDevice(S->device);
-    return add_graph_impl(S, V, E, src, true, graph_id);
+    const int32_t* d[4] = {d_fw_beg, d_fw_adj, d_bw_beg, d_bw_adj};
+    return register_device(S, V, E, d, graph_id);
 }
 
 int ispan_server_remove_graph(ispan_server* S, int64_t graph_id) {
@@ -391,11 +559,8 @@ int ispan_server_remove_graph(ispan_server* S, int64_t graph_id) {
     auto it = S->graphs.find(graph_id);
     if (it == S->graphs.end() || it->second.removed) return ISPAN_ERR_NOTFOUND;
     it->second.removed = true;
-    if (it->second.pending == 0) {
-        S->bytes_graphs -= it->second.bytes;
-        free_graph_mem(it->second);
-        S->graphs.erase(it);
-    }
+    S->bytes_live -= arena_bytes(it->second.V, it->second.E);
+    if (it->second.pending == 0) trim_arena(S);
     return ISPAN_OK;
 }
 
@@ -422,26 +587,43 @@ int ispan_server_wait(ispan_server* S, int64_t request_id, int32_t* labels, ispa
                       int64_t timeout_ms) {
     if (!S) return ISPAN_ERR_ARG;
     std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(request_id);
-    if (it == S->reqs.end()) return ISPAN_ERR_NOTFOUND;
-    auto ready = [S, request_id] { return S->reqs[request_id].done; };
-    if (timeout_ms < 0) S->cv_done.wait(lk, ready);
-    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return ISPAN_ERR_TIMEOUT;
-    Request& r = S->reqs[request_id];
-    if (info) {
-        info->submit_ms = r.t_submit;
-        info->start_ms = r.t_start;
-        info->done_ms = r.t_done;
-        info->latency_ms = r.t_done - r.t_submit;
-        info->batch = r.batch;
-        info->batch_graphs = r.batch_graphs;
-        info->status = r.status;
-    }
-    const int status = r.status;
-    if (status == ISPAN_OK && labels && !r.labels.empty())
-        memcpy(labels, r.labels.data(), r.labels.size() * 4);
+    int rc = wait_done(S, lk, request_id, timeout_ms);
+    if (rc) return rc;
+    Request r = S->reqs[request_id];
     S->reqs.erase(request_id);
-    return status;
+    fill_info(r, info);
+    rc = r.status;
+    if (rc == ISPAN_OK && labels && r.V > 0) {
+        lk.unlock();  // the buffer stays referenced until the copy is done
+        if (hipSetDevice(S->device) != hipSuccess ||
+            hipMemcpy(labels, r.d_labels, (size_t)r.V * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = ISPAN_ERR_HIP;
+        lk.lock();
+    }
+    drop_ref(S, r);
+    return rc;
+}
+
+int ispan_server_wait_device(ispan_server* S, int64_t request_id, const int32_t** d_labels, ispan_request_info* info,
+                             int64_t timeout_ms) {
+    if (!S || !d_labels) return ISPAN_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    const int rc = wait_done(S, lk, request_id, timeout_ms);
+    if (rc) return rc;
+    const Request& r = S->reqs[request_id];
+    fill_info(r, info);
+    *d_labels = r.d_labels;
+    return r.status;
+}
+
+int ispan_server_release(ispan_server* S, int64_t request_id) {
+    if (!S) return ISPAN_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(request_id);
+    if (it == S->reqs.end() || !it->second.done) return ISPAN_ERR_NOTFOUND;
+    drop_ref(S, it->second);
+    S->reqs.erase(it);
+    return ISPAN_OK;
 }
 
 int ispan_server_stats_get(ispan_server* S, ispan_server_stats* out) {
@@ -450,9 +632,12 @@ int ispan_server_stats_get(ispan_server* S, ispan_server_stats* out) {
     memset(out, 0, sizeof(*out));
     out->requests_done = (int64_t)S->latencies.size();
     out->batches = S->batches;
-    out->graphs_resident = (int64_t)S->graphs.size();
-    out->bytes_resident = S->bytes_graphs;
+    int64_t live = 0;
+    for (const auto& kv : S->graphs) live += !kv.second.removed;
+    out->graphs_resident = live;
+    out->bytes_resident = S->bytes_live;
     out->bytes_budget = S->budget;
+    out->bytes_device = bytes_used(S);
     out->p50_ms = percentile(S->latencies, 0.50);
     out->p99_ms = percentile(S->latencies, 0.99);
     double sum = 0, mx = 0;

@@ -86,9 +86,7 @@ struct ispan_ctx {
         int64_t E = 0;
         int32_t* csr[4] = {nullptr, nullptr, nullptr, nullptr};  // fw_beg, fw_adj, bw_beg, bw_adj
         int32_t* first[2] = {nullptr, nullptr};
-        int64_t* goff = nullptr;  // graph offsets inside the union [ng + 1]
-        int32_t* roots = nullptr;
-        int32_t* gmin = nullptr;
+        BatchRanges rb;  // its graphs' ranges inside the union, labels into batch_labels
     };
     std::vector<BatchGroup> groups;
     int32_t batch_n = -1;
@@ -97,13 +95,13 @@ struct ispan_ctx {
     bool batch_solved = false;
     const int32_t* batch_roots = nullptr;  // set while one group is solved
     int32_t batch_nroots = 0;
-    // reusable union workspace of batch_solve_tables (the server's batches)
-    BatchGroup ws;
-    int32_t ws_capV = -1, ws_capN = -1;
-    int64_t ws_capE = -1;
-    int64_t* ws_off = nullptr;             // v_off, e_off [2 * (capN + 1)]
-    const int32_t** ws_ptrs = nullptr;     // 4 pointer tables [4 * capN]
-    int32_t* ws_bad = nullptr;
+    const uint64_t* batch_sel = nullptr;  // set while a selection of an arena is solved
+    // ranges and selection bitmap of solve_arena (the server's batches)
+    BatchRanges ws_rb;
+    int32_t ws_capN = -1;
+    int64_t ws_capC = -1;
+    uint64_t* sel = nullptr;
+    int64_t sel_cap = -1;
     // kernel profiling (params.timing >= 2)
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -170,13 +168,24 @@ void free_solve_buffers(ispan_ctx* ctx) {
     ctx->have_solution = false;
 }
 
+void free_ranges(BatchRanges& r) {
+    dfree(r.vb);
+    dfree(r.ve);
+    dfree(r.cstart);
+    dfree(r.out);
+    dfree(r.cp);
+    dfree(r.ci);
+    dfree(r.roots);
+    dfree(r.gmin);
+    r.n = 0;
+    r.nchunks = 0;
+}
+
 void free_batch(ispan_ctx* ctx) {
     for (auto& g : ctx->groups) {
         for (auto& p : g.csr) dfree(p);
         for (auto& p : g.first) dfree(p);
-        dfree(g.goff);
-        dfree(g.roots);
-        dfree(g.gmin);
+        free_ranges(g.rb);
     }
     ctx->groups.clear();
     dfree(ctx->batch_labels);
@@ -188,17 +197,11 @@ void free_batch(ispan_ctx* ctx) {
 }
 
 void free_workspace(ispan_ctx* ctx) {
-    for (auto& p : ctx->ws.csr) dfree(p);
-    for (auto& p : ctx->ws.first) dfree(p);
-    dfree(ctx->ws.goff);
-    dfree(ctx->ws.roots);
-    dfree(ctx->ws.gmin);
-    dfree(ctx->ws_off);
-    if (ctx->ws_ptrs) (void)hipFree((void*)ctx->ws_ptrs);
-    ctx->ws_ptrs = nullptr;
-    dfree(ctx->ws_bad);
-    ctx->ws_capV = ctx->ws_capN = -1;
-    ctx->ws_capE = -1;
+    free_ranges(ctx->ws_rb);
+    ctx->ws_capN = -1;
+    ctx->ws_capC = -1;
+    dfree(ctx->sel);
+    ctx->sel_cap = -1;
 }
 
 void free_graph(ispan_ctx* ctx) {
@@ -448,6 +451,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.switch_rule = P->switch_rule;
         ga.roots = ctx->batch_roots;
         ga.n_roots = ctx->batch_nroots;
+        ga.sel = ctx->batch_sel;
         ga.rest = 1;
         ga.trim_times = P->trim_times;
         ga.theta = P->theta;
@@ -976,18 +980,24 @@ int ispan_solve_stats(ispan_ctx* ctx, ispan_stats* st) {
 
 namespace {
 
-void use_group(ispan_ctx* ctx, const ispan_ctx::BatchGroup& g) {
-    ctx->fwb = g.csr[0];
-    ctx->fwa = g.csr[1];
-    ctx->bwb = g.csr[2];
-    ctx->bwa = g.csr[3];
-    ctx->fw_first = g.first[0];
-    ctx->bw_first = g.first[1];
-    ctx->V = g.V;
-    ctx->E = g.E;
-    ctx->nwords = ((int64_t)g.V + 63) / 64;
-    ctx->batch_roots = g.roots;
-    ctx->batch_nroots = g.ng;
+// The CSR a batch solve runs on (a group's union or the server's arena).
+struct View {
+    const int32_t* csr[4];
+    const int32_t* first[2];
+    int32_t V;
+    int64_t E;
+};
+
+void use_view(ispan_ctx* ctx, const View& v) {
+    ctx->fwb = v.csr[0];
+    ctx->fwa = v.csr[1];
+    ctx->bwb = v.csr[2];
+    ctx->bwa = v.csr[3];
+    ctx->fw_first = const_cast<int32_t*>(v.first[0]);
+    ctx->bw_first = const_cast<int32_t*>(v.first[1]);
+    ctx->V = v.V;
+    ctx->E = v.E;
+    ctx->nwords = ((int64_t)v.V + 63) / 64;
 }
 
 void drop_group_view(ispan_ctx* ctx) {
@@ -998,6 +1008,83 @@ void drop_group_view(ispan_ctx* ctx) {
     ctx->nwords = 0;
     ctx->batch_roots = nullptr;
     ctx->batch_nroots = 0;
+    ctx->batch_sel = nullptr;
+}
+
+View group_view(const ispan_ctx::BatchGroup& g) {
+    return View{{g.csr[0], g.csr[1], g.csr[2], g.csr[3]}, {g.first[0], g.first[1]}, g.V, g.E};
+}
+
+// Device arrays of n graph ranges (grown to capacity *capN graphs / *capC
+// chunks when given, else sized exactly) and their upload.
+int ranges_upload(ispan_ctx* ctx, BatchRanges& r, int32_t* capN, int64_t* capC, int32_t n, const int64_t* vb,
+                  const int64_t* ve, int32_t* const* out) {
+    std::vector<int64_t> cs((size_t)n + 1, 0);
+    for (int32_t g = 0; g < n; ++g) cs[g + 1] = cs[g] + (ve[g] - vb[g] + kBatchChunk - 1) / kBatchChunk;
+    const int64_t nc = cs[n];
+    const bool growN = !capN || n > *capN, growC = !capC || nc > *capC;
+    if (growN) {
+        for (auto* p : {&r.vb, &r.ve, &r.cstart}) dfree(*p);
+        dfree(r.out);
+        dfree(r.roots);
+        dfree(r.gmin);
+        const size_t m = (size_t)std::max(n, 1) + 1;
+        CK(hipMalloc(&r.vb, m * 8));
+        CK(hipMalloc(&r.ve, m * 8));
+        CK(hipMalloc(&r.cstart, m * 8));
+        CK(hipMalloc(&r.out, m * sizeof(int32_t*)));
+        CK(hipMalloc(&r.roots, m * 4));
+        CK(hipMalloc(&r.gmin, m * 4));
+        if (capN) *capN = n;
+    }
+    if (growC) {
+        dfree(r.cp);
+        dfree(r.ci);
+        const size_t m = (size_t)std::max<int64_t>(nc, 1);
+        CK(hipMalloc(&r.cp, m * 8));
+        CK(hipMalloc(&r.ci, m * 4));
+        if (capC) *capC = nc;
+    }
+    hipStream_t s = ctx->stream;
+    if (n > 0) {
+        CK(hipMemcpyAsync(r.vb, vb, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(r.ve, ve, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync((void*)r.out, out, (size_t)n * sizeof(int32_t*), hipMemcpyHostToDevice, s));
+    }
+    CK(hipMemcpyAsync(r.cstart, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, s));
+    CK(hipStreamSynchronize(s));  // the host arrays may go after this
+    r.n = n;
+    r.nchunks = nc;
+    return ISPAN_OK;
+}
+
+// One pass of the pipeline over v, seeded with the pivot of every range in
+// r, each range's local labels into its output.  sel (optional) limits the
+// solve to the ranges' vertices.
+int solve_view(ispan_ctx* ctx, const View& v, const BatchRanges& r, const uint64_t* sel, const ispan_params* P,
+               ispan_stats* st) {
+    if (v.V == 0 || r.n == 0) return ISPAN_OK;
+    use_view(ctx, v);
+    ctx->batch_roots = r.roots;
+    ctx->batch_nroots = r.n;
+    ctx->batch_sel = sel;
+    int rc = ISPAN_OK;
+    hipError_t e = launch_batch_roots(v.csr[0], v.csr[2], r, P->pivot_wrap32, ctx->stream);
+    if (e != hipSuccess) {
+        ctx->err = std::string("launch_batch_roots: ") + hipGetErrorString(e);
+        rc = ISPAN_ERR_HIP;
+    }
+    if (!rc) rc = solve_impl(ctx, P, nullptr, st);
+    if (!rc) {
+        e = launch_batch_finish(ctx->lab, ctx->bvis, r, ctx->stream);
+        if (e != hipSuccess) {
+            ctx->err = std::string("launch_batch_finish: ") + hipGetErrorString(e);
+            rc = ISPAN_ERR_HIP;
+        }
+    }
+    drop_group_view(ctx);
+    ctx->have_solution = false;
+    return rc;
 }
 
 // Builds every group of a batch; on failure the caller frees the batch.
@@ -1026,6 +1113,7 @@ int batch_build(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* 
         dfree(d_bad);
         return rc;
     }
+    hip(hipMalloc(&ctx->batch_labels, (size_t)std::max<int64_t>(v_off[n], 1) * 4), "hipMalloc");
     for (int32_t g0 = 0; g0 < n && rc == ISPAN_OK;) {
         int32_t g1 = g0 + 1;
         while (g1 < n && e_off[g1 + 1] - e_off[g0] <= cap && v_off[g1 + 1] - v_off[g0] <= INT32_MAX - 64) ++g1;
@@ -1040,16 +1128,17 @@ int batch_build(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* 
         maxV = std::max(maxV, G.V);
         maxE = std::max(maxE, G.E);
         const size_t nb = ((size_t)G.V + 1) * 4, na = (size_t)std::max<int64_t>(G.E, 1) * 4;
-        std::vector<int64_t> goff(G.ng + 1);
-        for (int32_t i = 0; i <= G.ng; ++i) goff[i] = v_off[g0 + i] - G.vbase;
+        std::vector<int64_t> vb(G.ng), ve(G.ng);
+        std::vector<int32_t*> outp(G.ng);
+        for (int32_t i = 0; i < G.ng; ++i) {
+            vb[i] = v_off[g0 + i] - G.vbase;
+            ve[i] = v_off[g0 + i + 1] - G.vbase;
+            outp[i] = ctx->batch_labels + v_off[g0 + i];
+        }
         if (!hip(hipMalloc(&G.csr[0], nb), "hipMalloc") || !hip(hipMalloc(&G.csr[1], na), "hipMalloc") ||
             !hip(hipMalloc(&G.csr[2], nb), "hipMalloc") || !hip(hipMalloc(&G.csr[3], na), "hipMalloc") ||
             !hip(hipMalloc(&G.first[0], (size_t)std::max(G.V, 1) * 4), "hipMalloc") ||
             !hip(hipMalloc(&G.first[1], (size_t)std::max(G.V, 1) * 4), "hipMalloc") ||
-            !hip(hipMalloc(&G.goff, goff.size() * 8), "hipMalloc") ||
-            !hip(hipMalloc(&G.roots, (size_t)G.ng * 4), "hipMalloc") ||
-            !hip(hipMalloc(&G.gmin, (size_t)G.ng * 4), "hipMalloc") ||
-            !hip(hipMemcpyAsync(G.goff, goff.data(), goff.size() * 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
             !hip(launch_batch_union(fwb_cat, fwa_cat, bwb_cat, bwa_cat, d_off, d_off + n + 1, g0, G.ng, G.V, G.E,
                                     G.csr[0], G.csr[1], G.csr[2], G.csr[3], d_bad, s),
                  "launch_batch_union"))
@@ -1064,9 +1153,11 @@ int batch_build(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* 
             rc = ISPAN_ERR_CSR;
             break;
         }
-        use_group(ctx, G);
+        use_view(ctx, group_view(G));
         rc = check_bound(ctx, P);
         drop_group_view(ctx);
+        if (rc) break;
+        rc = ranges_upload(ctx, G.rb, nullptr, nullptr, G.ng, vb.data(), ve.data(), outp.data());
         if (rc) break;
         // first arrays only once the offsets are known to be in range
         if (G.V > 0 && (!hip(launch_first(G.csr[0], G.csr[1], G.V, G.first[0], ctx->grid, s), "launch_first") ||
@@ -1074,8 +1165,6 @@ int batch_build(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* 
             break;
         g0 = g1;
     }
-    if (rc == ISPAN_OK)
-        hip(hipMalloc(&ctx->batch_labels, (size_t)std::max<int64_t>(v_off[n], 1) * 4), "hipMalloc");
     if (rc == ISPAN_OK) rc = ensure_buffers(ctx, maxV, maxE);
     (void)hipStreamSynchronize(s);
     dfree(d_off);
@@ -1125,93 +1214,28 @@ int check_csr_device(int32_t V, int64_t E, const int32_t* fwb, const int32_t* fw
     return rc;
 }
 
-int64_t batch_workspace_bytes(int64_t V, int64_t E) {
-    // union CSR pair + first arrays, and the per-vertex solve state
-    return 8 * E + 16 * (V + 1) + 80 * (V + 1) + 2 * giant_hub_capacity(E) * (int64_t)sizeof(HubItem);
+int64_t solve_state_bytes(int64_t V, int64_t E) {
+    // per-vertex solve state (labels, bitmaps, queues, WCC and mice arrays)
+    // and the hub work items of both kinds of traversal
+    return 80 * (V + 1) + 2 * (giant_hub_capacity(E) + 2 * (E / kHubChunk) + 64) * (int64_t)sizeof(HubItem);
 }
 
-int batch_solve_tables(ispan_ctx* ctx, int32_t n, const int64_t* v_off, const int64_t* e_off,
-                       const int32_t* const* fwb, const int32_t* const* fwa, const int32_t* const* bwb,
-                       const int32_t* const* bwa, const ispan_params* P, int32_t* d_out, ispan_stats* st) {
-    if (!ctx || n < 1 || v_off[0] != 0 || e_off[0] != 0) return ISPAN_ERR_ARG;
-    const int64_t Vu = v_off[n], Eu = e_off[n];
-    if (Vu > INT32_MAX - 64 || Eu > INT32_MAX) return ISPAN_ERR_ARG;
+int solve_arena(ispan_ctx* ctx, const ArenaView& av, int32_t n, const int64_t* vb, const int64_t* ve,
+                int32_t* const* out, const ispan_params* P, ispan_stats* st) {
+    if (!ctx || n < 1 || av.V < 0) return ISPAN_ERR_ARG;
     if (ctx->batch_n >= 0 || ctx->V >= 0) free_graph(ctx);
-    hipStream_t s = ctx->stream;
-    ispan_ctx::BatchGroup& G = ctx->ws;
-    if (Vu > ctx->ws_capV || Eu > ctx->ws_capE) {
-        for (auto& p : G.csr) dfree(p);
-        for (auto& p : G.first) dfree(p);
-        const size_t nb = ((size_t)Vu + 1) * 4, na = (size_t)std::max<int64_t>(Eu, 1) * 4;
-        CK(hipMalloc(&G.csr[0], nb));
-        CK(hipMalloc(&G.csr[1], na));
-        CK(hipMalloc(&G.csr[2], nb));
-        CK(hipMalloc(&G.csr[3], na));
-        CK(hipMalloc(&G.first[0], nb));
-        CK(hipMalloc(&G.first[1], nb));
-        ctx->ws_capV = (int32_t)Vu;
-        ctx->ws_capE = Eu;
+    int rc = ranges_upload(ctx, ctx->ws_rb, &ctx->ws_capN, &ctx->ws_capC, n, vb, ve, out);
+    if (rc) return rc;
+    const int64_t nwords = ((int64_t)av.V + 63) / 64;
+    if (nwords > ctx->sel_cap) {
+        dfree(ctx->sel);
+        CK(hipMalloc(&ctx->sel, (size_t)std::max<int64_t>(nwords, 1) * 8));
+        ctx->sel_cap = nwords;
     }
-    if (n > ctx->ws_capN) {
-        dfree(G.goff);
-        dfree(G.roots);
-        dfree(G.gmin);
-        dfree(ctx->ws_off);
-        if (ctx->ws_ptrs) (void)hipFree((void*)ctx->ws_ptrs);
-        ctx->ws_ptrs = nullptr;
-        CK(hipMalloc(&G.goff, ((size_t)n + 1) * 8));
-        CK(hipMalloc(&G.roots, (size_t)n * 4));
-        CK(hipMalloc(&G.gmin, (size_t)n * 4));
-        CK(hipMalloc(&ctx->ws_off, 2 * ((size_t)n + 1) * 8));
-        CK(hipMalloc((void**)&ctx->ws_ptrs, 4 * (size_t)n * sizeof(void*)));
-        if (!ctx->ws_bad) CK(hipMalloc(&ctx->ws_bad, 4));
-        ctx->ws_capN = n;
-    }
-    G.g0 = 0;
-    G.ng = n;
-    G.vbase = 0;
-    G.ebase = 0;
-    G.V = (int32_t)Vu;
-    G.E = Eu;
-    std::vector<const int32_t*> tabs(4 * (size_t)n);
-    for (int32_t g = 0; g < n; ++g) {
-        tabs[g] = fwb[g];
-        tabs[n + g] = fwa[g];
-        tabs[2 * (size_t)n + g] = bwb[g];
-        tabs[3 * (size_t)n + g] = bwa[g];
-    }
-    CK(hipMemcpyAsync(ctx->ws_off, v_off, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
-    CK(hipMemcpyAsync(ctx->ws_off + n + 1, e_off, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
-    CK(hipMemcpyAsync(G.goff, v_off, ((size_t)n + 1) * 8, hipMemcpyHostToDevice, s));
-    CK(hipMemcpyAsync((void*)ctx->ws_ptrs, tabs.data(), tabs.size() * sizeof(void*), hipMemcpyHostToDevice, s));
-    // the graphs were checked when they were registered
-    CK(launch_batch_union_ptrs(ctx->ws_ptrs, ctx->ws_ptrs + n, ctx->ws_ptrs + 2 * (size_t)n,
-                               ctx->ws_ptrs + 3 * (size_t)n, ctx->ws_off, ctx->ws_off + n + 1, n, Vu, Eu, G.csr[0],
-                               G.csr[1], G.csr[2], G.csr[3], ctx->ws_bad, s));
-    if (Vu > 0) {
-        CK(launch_first(G.csr[0], G.csr[1], (int32_t)Vu, G.first[0], ctx->grid, s));
-        CK(launch_first(G.csr[2], G.csr[3], (int32_t)Vu, G.first[1], ctx->grid, s));
-    }
+    CK(launch_batch_select(ctx->ws_rb, ctx->sel, nwords, ctx->stream));
+    const View v{{av.fwb, av.fwa, av.bwb, av.bwa}, {av.fw_first, av.bw_first}, av.V, av.E};
     ispan_stats local{};
-    int rc = ISPAN_OK;
-    if (Vu > 0) {
-        use_group(ctx, G);
-        hipError_t e = launch_batch_roots(G.csr[0], G.csr[2], G.goff, n, P->pivot_wrap32, G.roots, s);
-        if (e != hipSuccess) {
-            ctx->err = std::string("launch_batch_roots: ") + hipGetErrorString(e);
-            rc = ISPAN_ERR_HIP;
-        }
-        if (!rc) rc = solve_impl(ctx, P, nullptr, &local);
-        if (!rc) {
-            e = launch_batch_finish(ctx->lab, ctx->bvis, G.goff, G.roots, n, Vu, G.gmin, d_out, s);
-            if (e != hipSuccess) {
-                ctx->err = std::string("launch_batch_finish: ") + hipGetErrorString(e);
-                rc = ISPAN_ERR_HIP;
-            }
-        }
-        drop_group_view(ctx);
-        ctx->have_solution = false;
-    }
+    rc = solve_view(ctx, v, ctx->ws_rb, ctx->sel, P, &local);
     if (st) *st = local;
     return rc;
 }
@@ -1261,26 +1285,8 @@ int ispan_batch_solve(ispan_ctx* ctx, const ispan_params* params, ispan_stats* s
     ispan_stats tot{};
     ctx->batch_solved = false;
     for (const auto& G : ctx->groups) {
-        if (G.V == 0) continue;
-        use_group(ctx, G);
-        int rc = ISPAN_OK;
-        hipError_t e = launch_batch_roots(G.csr[0], G.csr[2], G.goff, G.ng, P->pivot_wrap32, G.roots, ctx->stream);
-        if (e != hipSuccess) {
-            ctx->err = std::string("launch_batch_roots: ") + hipGetErrorString(e);
-            rc = ISPAN_ERR_HIP;
-        }
         ispan_stats st{};
-        if (!rc) rc = solve_impl(ctx, P, nullptr, &st);
-        if (!rc) {
-            e = launch_batch_finish(ctx->lab, ctx->bvis, G.goff, G.roots, G.ng, G.V, G.gmin,
-                                    ctx->batch_labels + G.vbase, ctx->stream);
-            if (e != hipSuccess) {
-                ctx->err = std::string("launch_batch_finish: ") + hipGetErrorString(e);
-                rc = ISPAN_ERR_HIP;
-            }
-        }
-        drop_group_view(ctx);
-        ctx->have_solution = false;
+        const int rc = solve_view(ctx, group_view(G), G.rb, nullptr, P, &st);
         if (rc) return rc;
         add_stats(tot, st);
     }

@@ -127,3 +127,62 @@ def test_bad_inputs():
         with pytest.raises(I.IspanError) as e:
             srv.wait(777)
         assert e.value.code == -11
+
+
+def test_labels_in_device_memory(pool):
+    from ispan_amd import hiprt
+    graphs, ref = pool
+    with I.Server(0) as srv:
+        ids = [srv.add_graph(host(g)) for g in graphs[:5]]
+        rids = [srv.submit(gid) for gid in ids]
+        for k, rid in enumerate(rids):
+            ptr, info = srv.wait_device(rid)
+            V = len(graphs[k].fw_beg) - 1
+            out = np.empty(V, np.int32)
+            hiprt.rt().hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)
+            srv.release(rid)
+            assert np.array_equal(out, ref[k])
+        with pytest.raises(I.IspanError) as e:
+            srv.release(rids[0])
+        assert e.value.code == -11
+
+
+def test_arena_grows_while_serving(pool):
+    graphs, ref = pool
+    with I.Server(0) as srv:
+        first = srv.add_graph(host(graphs[0]))
+        stop, errors, done = [False], [], []
+
+        def client():
+            try:
+                while not stop[0]:
+                    lab, _ = srv.wait(srv.submit(first))
+                    done.append(np.array_equal(lab, ref[0]))
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(e)
+
+        t = threading.Thread(target=client)
+        t.start()
+        try:
+            ids = [srv.add_graph(host(g)) for g in graphs[1:]]  # each one may grow the arena
+        finally:
+            stop[0] = True
+            t.join()
+        assert not errors and done and all(done)
+        for gid, r in zip(ids, ref[1:]):
+            lab, _ = srv.wait(srv.submit(gid))
+            assert np.array_equal(lab, r)
+
+
+def test_arena_space_reused_after_removal(pool):
+    graphs, ref = pool
+    with I.Server(0) as srv:
+        a = srv.add_graph(host(graphs[2]))
+        b = srv.add_graph(host(graphs[4]))
+        used = srv.stats().bytes_device
+        srv.remove_graph(b)
+        c = srv.add_graph(host(graphs[3]))  # smaller than b: fits where b was
+        assert srv.stats().bytes_device == used
+        for gid, r in ((a, ref[2]), (c, ref[3])):
+            lab, _ = srv.wait(srv.submit(gid))
+            assert np.array_equal(lab, r)
