@@ -241,6 +241,9 @@ int ispan_server_add_graph_device(ispan_server* srv, int32_t V, int64_t E, const
 int ispan_server_remove_graph(ispan_server* srv, int64_t graph_id);
 /* Queue a solve of graph_id; returns at once. */
 int ispan_server_submit(ispan_server* srv, int64_t graph_id, int64_t* request_id);
+/* Queue n solves at once (they are in the queue together, so they can share
+ * a batch); nothing is queued if an id is unknown. */
+int ispan_server_submit_many(ispan_server* srv, int32_t n, const int64_t* graph_ids, int64_t* request_ids);
 /* Wait for a request (timeout_ms < 0: no limit), copy its labels (int32[V])
  * to host memory if labels is not NULL, and forget it.  Returns its status. */
 int ispan_server_wait(ispan_server* srv, int64_t request_id, int32_t* labels, ispan_request_info* info,

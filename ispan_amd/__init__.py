@@ -39,7 +39,7 @@ EXPORTS = [
     "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
     "ispan_convert_edge_list", "ispan_batch_bind_device", "ispan_batch_groups", "ispan_batch_solve",
     "ispan_batch_labels_download", "ispan_batch_labels_device", "ispan_server_create", "ispan_server_destroy",
-    "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit",
+    "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit", "ispan_server_submit_many",
     "ispan_server_wait", "ispan_server_wait_device", "ispan_server_release", "ispan_server_stats_get",
     "ispan_server_reset_stats", "ispan_server_last_error",
 ]
@@ -180,6 +180,7 @@ def lib():
         "ispan_server_remove_graph": (C.c_int, [_vp, C.c_int64]),
         "ispan_server_submit": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_int64)]),
         "ispan_server_wait": (C.c_int, [_vp, C.c_int64, _vp, C.POINTER(RequestInfo), C.c_int64]),
+        "ispan_server_submit_many": (C.c_int, [_vp, C.c_int32, _vp, _vp]),
         "ispan_server_wait_device": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(RequestInfo), C.c_int64]),
         "ispan_server_release": (C.c_int, [_vp, C.c_int64]),
         "ispan_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
@@ -475,6 +476,14 @@ class Server:
         self._err(lib().ispan_server_submit(self._srv, gid, C.byref(rid)))
         self._rid_V[rid.value] = self._V[gid]
         return rid.value
+
+    def submit_many(self, gids) -> list[int]:
+        g = np.ascontiguousarray(gids, np.int64)
+        r = np.empty(len(g), np.int64)
+        self._err(lib().ispan_server_submit_many(self._srv, len(g), _ptr(g), _ptr(r)))
+        for rid, gid in zip(r.tolist(), g.tolist()):
+            self._rid_V[rid] = self._V[gid]
+        return r.tolist()
 
     def wait(self, rid: int, timeout_ms: int = -1, want_labels: bool = True):
         """-> (labels or None, RequestInfo)."""

@@ -17,7 +17,7 @@ using namespace dev;
 
 namespace {
 
-constexpr int64_t kChunk = kBatchChunk;  // work per block (vertices or edges)
+constexpr int64_t kUnionChunk = 1 << 14;  // union building: vertices or edges per block
 
 // Largest g with off[g] <= x, for off[0..n] ascending (x < off[n]).
 __device__ __forceinline__ int32_t find_graph(const int64_t* off, int32_t n, int64_t x) {
@@ -50,9 +50,9 @@ template <class Src>
 __global__ void k_union_beg(Src src, const int64_t* __restrict__ v_off, const int64_t* __restrict__ e_off, int32_t g0,
                             int32_t ng, int32_t* __restrict__ ubeg, int32_t* bad) {
     const int64_t vb = v_off[g0], Vu = v_off[g0 + ng] - vb, eb = e_off[g0];
-    const int64_t c0 = (int64_t)blockIdx.x * kChunk;
+    const int64_t c0 = (int64_t)blockIdx.x * kUnionChunk;
     if (c0 >= Vu) return;
-    const int64_t c1 = min(c0 + kChunk, Vu);
+    const int64_t c1 = min(c0 + kUnionChunk, Vu);
     int32_t g = find_graph(v_off + g0, ng, vb + c0 + threadIdx.x < vb + c1 ? vb + c0 + threadIdx.x : vb + c0) + g0;
     for (int64_t u = c0 + threadIdx.x; u < c1; u += blockDim.x) {
         while (vb + u >= v_off[g + 1]) ++g;
@@ -76,9 +76,9 @@ template <class Src>
 __global__ void k_union_adj(Src src, const int64_t* __restrict__ v_off, const int64_t* __restrict__ e_off, int32_t g0,
                             int32_t ng, int32_t* __restrict__ uadj, int32_t* bad) {
     const int64_t vb = v_off[g0], eb = e_off[g0], Eu = e_off[g0 + ng] - eb;
-    const int64_t c0 = (int64_t)blockIdx.x * kChunk;
+    const int64_t c0 = (int64_t)blockIdx.x * kUnionChunk;
     if (c0 >= Eu) return;
-    const int64_t c1 = min(c0 + kChunk, Eu);
+    const int64_t c1 = min(c0 + kUnionChunk, Eu);
     const int64_t first = c0 + threadIdx.x < c1 ? c0 + threadIdx.x : c0;
     int32_t g = find_graph(e_off + g0, ng, eb + first) + g0;
     int any_bad = 0;
@@ -93,7 +93,7 @@ __global__ void k_union_adj(Src src, const int64_t* __restrict__ v_off, const in
 }
 
 // Graph ranges of a batch: graph g owns vertices [vb[g], ve[g]) of the CSR
-// being solved, split into kChunk-vertex chunks; cstart[g] is the index of
+// being solved, split into kBatchChunk-vertex chunks; cstart[g] is the index of
 // its first chunk (cstart[n] chunks in all).  One block per chunk, so the
 // work follows the vertex count, not the number of graphs.
 struct Ranges {
@@ -105,8 +105,8 @@ struct Ranges {
 
 __device__ __forceinline__ void chunk_of(const Ranges& R, int64_t b, int32_t& g, int64_t& v0, int64_t& v1) {
     g = find_graph(R.cstart, R.n, b);
-    v0 = R.vb[g] + (b - R.cstart[g]) * kChunk;
-    v1 = min(v0 + kChunk, R.ve[g]);
+    v0 = R.vb[g] + (b - R.cstart[g]) * kBatchChunk;
+    v1 = min(v0 + kBatchChunk, R.ve[g]);
 }
 
 // Selection bitmap of the batch's vertices (sel cleared beforehand).
@@ -236,7 +236,7 @@ __global__ void k_arena_append(const int32_t* __restrict__ beg, const int32_t* _
     for (int64_t k = t; k < E; k += stride) aadj[eb + k] = (int32_t)(adj[k] + vb);
 }
 
-int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kChunk - 1) / kChunk); }
+int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kUnionChunk - 1) / kUnionChunk); }
 
 }  // namespace
 

@@ -151,7 +151,7 @@ hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, co
                               int32_t* ubwa, int32_t* bad, hipStream_t s);
 // Graph ranges of one batch solve (device arrays, see batch.hip): graph g
 // owns vertices [vb[g], ve[g]) of the CSR being solved; its local labels go
-// to out[g]; cstart is the prefix of its kChunk-vertex chunk counts.
+// to out[g]; cstart is the prefix of its kBatchChunk-vertex chunk counts.
 struct BatchRanges {
     int32_t n = 0;
     int64_t nchunks = 0;
@@ -164,7 +164,7 @@ struct BatchRanges {
     int32_t* roots = nullptr;
     int32_t* gmin = nullptr;
 };
-constexpr int64_t kBatchChunk = 1 << 16;  // vertices per chunk (kChunk in batch.hip)
+constexpr int64_t kBatchChunk = 4096;  // vertices per chunk: one block each
 hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const BatchRanges& r, int32_t wrap32,
                               hipStream_t s);
 hipError_t launch_batch_finish(const int32_t* lab, const uint64_t* bvis, const BatchRanges& r, hipStream_t s);
@@ -223,7 +223,8 @@ hipError_t launch_mice_starts(const uint64_t* keys, int32_t n, int32_t* starts, 
 hipError_t launch_mice(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa, int32_t* lab,
                        int32_t* stamp, const uint64_t* keys, int32_t n, const int32_t* starts, Ctl* ctl,
                        int32_t* qbuf, int grid, hipStream_t s);
-hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, Ctl* ctl, int grid, hipStream_t s);
+hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, const uint64_t* sel, Ctl* ctl, int grid,
+                                   hipStream_t s);
 hipError_t launch_stats(const int32_t* fwb, const int32_t* bwb, const int32_t* lab, const uint64_t* fvis,
                         const uint64_t* bvis, int32_t V, Ctl* ctl, int grid, hipStream_t s);
 hipError_t launch_check_csr(const int32_t* beg, const int32_t* adj, int32_t V, int64_t E, Ctl* ctl, int grid,

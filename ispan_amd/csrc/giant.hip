@@ -777,7 +777,8 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         const uint64_t a0 = load_agent(&a.act[w]);
         const uint64_t g = traversed ? load_agent(&a.bvis[w]) : 0ull;
         const int64_t v = w * 64 + c.lane;
-        if (v < a.V) {
+        const uint64_t sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
+        if (v < a.V && ((sw >> c.lane) & 1ull)) {
             const bool gi = (g >> c.lane) & 1ull, ac = (a0 >> c.lane) & 1ull;
             st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (ac ? -1 : (int32_t)v));
         }

@@ -698,10 +698,12 @@ __global__ __launch_bounds__(kBlock) void k_mice(const int32_t* __restrict__ fwb
 }
 
 // ---------------------------------------------------------------- checks
-__global__ __launch_bounds__(kBlock) void k_count_unlabelled(const int32_t* __restrict__ lab, int32_t V, Ctl* ctl) {
+// sel (optional): only the selected vertices count.
+__global__ __launch_bounds__(kBlock) void k_count_unlabelled(const int32_t* __restrict__ lab, int32_t V,
+                                                             const uint64_t* __restrict__ sel, Ctl* ctl) {
     int c = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x)
-        c += lab[i] < 0;
+        c += (!sel || ((sel[i >> 6] >> (i & 63)) & 1ull)) && lab[i] < 0;
     c = wave_sum(c);
     if (lane_id() == 0 && c) atomicAdd(&ctl->unlabelled, c);
 }
@@ -861,8 +863,9 @@ hipError_t launch_mice(const int32_t* fwb, const int32_t* fwa, const int32_t* bw
     LAUNCH(k_mice, grid, fwb, fwa, bwb, bwa, lab, stamp, keys, n, starts, ctl, qbuf);
 }
 
-hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, Ctl* ctl, int grid, hipStream_t s) {
-    LAUNCH(k_count_unlabelled, grid, lab, V, ctl);
+hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, const uint64_t* sel, Ctl* ctl, int grid,
+                                   hipStream_t s) {
+    LAUNCH(k_count_unlabelled, grid, lab, V, sel, ctl);
 }
 
 hipError_t launch_check_csr(const int32_t* beg, const int32_t* adj, int32_t V, int64_t E, Ctl* ctl, int grid,

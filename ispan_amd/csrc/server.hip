@@ -583,6 +583,31 @@ int ispan_server_submit(ispan_server* S, int64_t graph_id, int64_t* request_id) 
     return ISPAN_OK;
 }
 
+int ispan_server_submit_many(ispan_server* S, int32_t n, const int64_t* graph_ids, int64_t* request_ids) {
+    if (!S || n < 0 || (n > 0 && (!graph_ids || !request_ids))) return ISPAN_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (int32_t i = 0; i < n; ++i) {
+            auto it = S->graphs.find(graph_ids[i]);
+            if (it == S->graphs.end() || it->second.removed) return ISPAN_ERR_NOTFOUND;
+        }
+        const double t = now_ms();
+        for (int32_t i = 0; i < n; ++i) {
+            Graph& g = S->graphs[graph_ids[i]];
+            g.pending += 1;
+            const int64_t id = S->next_req++;
+            Request& r = S->reqs[id];
+            r.graph = graph_ids[i];
+            r.V = g.V;
+            r.t_submit = t;
+            S->pending.push_back(id);
+            request_ids[i] = id;
+        }
+    }
+    S->cv_work.notify_one();
+    return ISPAN_OK;
+}
+
 int ispan_server_wait(ispan_server* S, int64_t request_id, int32_t* labels, ispan_request_info* info,
                       int64_t timeout_ms) {
     if (!S) return ISPAN_ERR_ARG;

@@ -102,6 +102,8 @@ struct ispan_ctx {
     int64_t ws_capC = -1;
     uint64_t* sel = nullptr;
     int64_t sel_cap = -1;
+    void* stage = nullptr;  // pinned staging of range uploads
+    size_t stage_bytes = 0;
     // kernel profiling (params.timing >= 2)
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -169,10 +171,9 @@ void free_solve_buffers(ispan_ctx* ctx) {
 }
 
 void free_ranges(BatchRanges& r) {
-    dfree(r.vb);
-    dfree(r.ve);
-    dfree(r.cstart);
-    dfree(r.out);
+    dfree(r.vb);  // one block: vb, ve, cstart, out
+    r.ve = r.cstart = nullptr;
+    r.out = nullptr;
     dfree(r.cp);
     dfree(r.ci);
     dfree(r.roots);
@@ -202,6 +203,9 @@ void free_workspace(ispan_ctx* ctx) {
     ctx->ws_capC = -1;
     dfree(ctx->sel);
     ctx->sel_cap = -1;
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
 }
 
 void free_graph(ispan_ctx* ctx) {
@@ -672,7 +676,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         rec(ctx, P, 8);
     }
     rec(ctx, P, 9);
-    CK(launch_count_unlabelled(ctx->lab, V, ctx->ctl, G, s));
+    CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
     if ((rc = readback(ctx, &syncs))) return rc;
     if (ctx->h_ctl->unlabelled) {
         ctx->err = std::to_string(ctx->h_ctl->unlabelled) + " vertices left unlabelled";
@@ -1016,43 +1020,55 @@ View group_view(const ispan_ctx::BatchGroup& g) {
 }
 
 // Device arrays of n graph ranges (grown to capacity *capN graphs / *capC
-// chunks when given, else sized exactly) and their upload.
+// chunks when given, else sized exactly) and their upload: vb, ve, cstart
+// and out packed into one pinned staging area, sent with one copy on the
+// context's stream (the solve that follows orders every later use of it).
 int ranges_upload(ispan_ctx* ctx, BatchRanges& r, int32_t* capN, int64_t* capC, int32_t n, const int64_t* vb,
                   const int64_t* ve, int32_t* const* out) {
-    std::vector<int64_t> cs((size_t)n + 1, 0);
-    for (int32_t g = 0; g < n; ++g) cs[g + 1] = cs[g] + (ve[g] - vb[g] + kBatchChunk - 1) / kBatchChunk;
-    const int64_t nc = cs[n];
+    const size_t m = (size_t)std::max(n, 1) + 1;
+    const size_t bytes = 4 * m * 8;  // vb, ve, cstart (int64) and out (pointers), m entries each
+    if (bytes > ctx->stage_bytes) {
+        CK(hipStreamSynchronize(ctx->stream));  // a pending copy may still read the old area
+        if (ctx->stage) (void)hipHostFree(ctx->stage);
+        ctx->stage = nullptr;
+        ctx->stage_bytes = 0;
+        CK(hipHostMalloc(&ctx->stage, bytes, hipHostMallocDefault));
+        ctx->stage_bytes = bytes;
+    } else {
+        CK(hipStreamSynchronize(ctx->stream));  // the previous upload has been read
+    }
+    int64_t* h = static_cast<int64_t*>(ctx->stage);
+    int64_t* hcs = h + 2 * m;
+    hcs[0] = 0;
+    for (int32_t g = 0; g < n; ++g) {
+        h[g] = vb[g];
+        h[m + g] = ve[g];
+        hcs[g + 1] = hcs[g] + (ve[g] - vb[g] + kBatchChunk - 1) / kBatchChunk;
+        reinterpret_cast<int32_t**>(h + 3 * m)[g] = out[g];
+    }
+    const int64_t nc = hcs[n];
     const bool growN = !capN || n > *capN, growC = !capC || nc > *capC;
     if (growN) {
-        for (auto* p : {&r.vb, &r.ve, &r.cstart}) dfree(*p);
-        dfree(r.out);
+        dfree(r.vb);  // vb owns the packed block: vb, ve, cstart, out
         dfree(r.roots);
         dfree(r.gmin);
-        const size_t m = (size_t)std::max(n, 1) + 1;
-        CK(hipMalloc(&r.vb, m * 8));
-        CK(hipMalloc(&r.ve, m * 8));
-        CK(hipMalloc(&r.cstart, m * 8));
-        CK(hipMalloc(&r.out, m * sizeof(int32_t*)));
+        CK(hipMalloc(&r.vb, bytes));
         CK(hipMalloc(&r.roots, m * 4));
         CK(hipMalloc(&r.gmin, m * 4));
         if (capN) *capN = n;
     }
+    r.ve = r.vb + m;
+    r.cstart = r.vb + 2 * m;
+    r.out = reinterpret_cast<int32_t**>(r.vb + 3 * m);
     if (growC) {
         dfree(r.cp);
         dfree(r.ci);
-        const size_t m = (size_t)std::max<int64_t>(nc, 1);
-        CK(hipMalloc(&r.cp, m * 8));
-        CK(hipMalloc(&r.ci, m * 4));
+        const size_t c = (size_t)std::max<int64_t>(nc, 1);
+        CK(hipMalloc(&r.cp, c * 8));
+        CK(hipMalloc(&r.ci, c * 4));
         if (capC) *capC = nc;
     }
-    hipStream_t s = ctx->stream;
-    if (n > 0) {
-        CK(hipMemcpyAsync(r.vb, vb, (size_t)n * 8, hipMemcpyHostToDevice, s));
-        CK(hipMemcpyAsync(r.ve, ve, (size_t)n * 8, hipMemcpyHostToDevice, s));
-        CK(hipMemcpyAsync((void*)r.out, out, (size_t)n * sizeof(int32_t*), hipMemcpyHostToDevice, s));
-    }
-    CK(hipMemcpyAsync(r.cstart, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, s));
-    CK(hipStreamSynchronize(s));  // the host arrays may go after this
+    CK(hipMemcpyAsync(r.vb, h, bytes, hipMemcpyHostToDevice, ctx->stream));
     r.n = n;
     r.nchunks = nc;
     return ISPAN_OK;

@@ -186,3 +186,18 @@ def test_arena_space_reused_after_removal(pool):
         for gid, r in ((a, ref[2]), (c, ref[3])):
             lab, _ = srv.wait(srv.submit(gid))
             assert np.array_equal(lab, r)
+
+
+def test_submit_many_shares_one_batch(pool):
+    graphs, ref = pool
+    with I.Server(0) as srv:
+        ids = [srv.add_graph(host(g)) for g in graphs]
+        rids = srv.submit_many(ids + ids[:3])  # the last three repeat graphs of the batch
+        for k, rid in enumerate(rids):
+            lab, info = srv.wait(rid)
+            assert info.batch_graphs == len(rids)
+            assert np.array_equal(lab, ref[k % len(ids)])
+        with pytest.raises(I.IspanError) as e:
+            srv.submit_many([ids[0], 999])
+        assert e.value.code == -11
+        assert srv.stats().requests_done == len(rids)
