@@ -173,15 +173,26 @@ __device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t en
         while (p < lim && !found) found = pred(adj[p++], self);
         scanned += p - p0;
     }
+    // the rest of a long list: the whole wave, 4 x 64 entries per step with
+    // their loads issued together (a hub's list would otherwise take one
+    // dependent round trip per 64 entries)
+    constexpr int kSteps = 4;
     uint64_t pend = __ballot(active && !found && p < end);
     while (pend) {
         const int l = __ffsll((unsigned long long)pend) - 1;
         const int32_t pp = __shfl(p, l, kWave), ee = __shfl(end, l, kWave), ss = __shfl(self, l, kWave);
         bool hit = false;
         int32_t qb = pp;
-        for (; qb < ee; qb += kWave) {
-            const int32_t q = qb + lane_id();
-            const bool h = q < ee && pred(adj[q], ss);
+        for (; qb < ee; qb += kSteps * kWave) {
+            int32_t a[kSteps];
+#pragma unroll
+            for (int k = 0; k < kSteps; ++k) {
+                const int32_t q = qb + k * kWave + lane_id();
+                a[k] = q < ee ? adj[q] : -1;
+            }
+            bool h = false;  // every predicate load issued, none waits on another
+#pragma unroll
+            for (int k = 0; k < kSteps; ++k) h |= pred(a[k] >= 0 ? a[k] : ss, ss) & (a[k] >= 0);
             if (__ballot(h)) {
                 hit = true;
                 break;
@@ -189,7 +200,7 @@ __device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t en
         }
         if (lane_id() == l) {
             found = hit;
-            scanned += min(qb + kWave, ee) - pp;
+            scanned += min(qb + kSteps * kWave, ee) - pp;
         }
         pend &= pend - 1;
     }

@@ -7,7 +7,9 @@
 // Every block runs the same control flow: after each barrier all blocks sum
 // the same per-block partial counts and take the same top-down/bottom-up
 // decision, so every block passes the same barriers.  Data written by other
-// blocks before a barrier is read with loads that bypass the per-CU cache.
+// blocks before a barrier is read with plain loads after it (every block
+// invalidates its caches when it passes a barrier); device-coherent loads are
+// kept for what other blocks change during the same phase.
 #include "common.h"
 #include "device_util.h"
 
@@ -321,7 +323,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     const int64_t i = u * kGBlock + c.tid;
                     int32_t start = 0, deg = 0;
                     if (i < n) {
-                        const int32_t v = load_agent(&qin[i]);
+                        const int32_t v = qin[i];
                         start = beg[v];
                         deg = beg[v + 1] - start;
                     }
@@ -357,8 +359,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         v = root;
                         s0 = beg[root] + (int32_t)(u - ntiles) * kGiantChunk;
                     } else {
-                        v = load_agent(&hin[u - ntiles].v);
-                        s0 = load_agent(&hin[u - ntiles].start);
+                        v = hin[u - ntiles].v;
+                        s0 = hin[u - ntiles].start;
                     }
                     const int32_t e1 = min(s0 + kGiantChunk, beg[v + 1]);
                     if (c.tid == 0) c.edges_td += e1 - s0;
@@ -387,12 +389,36 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             // a switch back to top-down needs no extra pass.
             constexpr int kU = 4;
             int scanned = 0;
+            // candidates whose first probe misses are parked in the wave's LDS
+            // buffer and scanned 64 at a time, one per lane, so their load
+            // chains overlap instead of idling the other 63 lanes
+            int nmiss = 0;  // wave-uniform
+            auto scan_misses = [&]() {
+                wave_sync();
+                for (int i0 = 0; i0 < nmiss; i0 += kWave) {
+                    const int i = i0 + c.lane;
+                    int32_t v = 0, p = 0, e = 0;
+                    if (i < nmiss) {
+                        v = wq.buf[i];
+                        p = rbeg[v] + 1;
+                        e = rbeg[v + 1];
+                    }
+                    const bool found = wave_any_edge(i < nmiss && p < e, p, e, v, radj, is_vis, scanned);
+                    if (found) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(vis + (v >> 6)), 1ull << (v & 63));
+                        my_found += 1;
+                        if (BW) gmin = min(gmin, v);
+                    }
+                }
+                nmiss = 0;
+                wave_sync();
+            };
             for (int64_t w0 = gwave * kU; w0 < a.nwords; w0 += total_waves * kU) {
                 // lane u < kU holds word w0 + u; the others read it by shuffle
                 uint64_t my_vw = 0, my_cand = 0;
                 if (c.lane < kU && w0 + c.lane < a.nwords) {
-                    my_vw = load_agent(&vis[w0 + c.lane]);
-                    my_cand = load_agent(&allow[w0 + c.lane]) & ~my_vw;
+                    my_vw = vis[w0 + c.lane];
+                    my_cand = allow[w0 + c.lane] & ~my_vw;
                     st_agent(&a.snap[w0 + c.lane], my_vw);
                 }
                 int32_t f[kU];
@@ -411,23 +437,21 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     const int64_t w = w0 + u;
                     const int32_t v = (int32_t)(w * 64 + c.lane);
                     const bool is_c = (cm >> c.lane) & 1ull;
-                    const bool hit = (hits >> u) & 1u;
-                    int32_t p = 0, e = 0;
-                    if (is_c) {
-                        scanned += 1;
-                        if (!hit) {
-                            p = rbeg[v] + 1;
-                            e = rbeg[v + 1];
-                        }
+                    const bool hit = is_c && ((hits >> u) & 1u);
+                    scanned += is_c;
+                    const uint64_t mm = __ballot(is_c && !hit && f[u] >= 0);  // misses with more list to scan
+                    if (mm) {
+                        if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
+                        if ((mm >> c.lane) & 1ull) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
+                        nmiss += __popcll(mm);
                     }
-                    const bool found = wave_any_edge(is_c && !hit, p, e, v, radj, is_vis, scanned) || (is_c && hit);
-                    const uint64_t fm = __ballot(found);
-                    const uint64_t vwu = __shfl(my_vw, u, kWave);
-                    if (fm && c.lane == 0) st_agent(&vis[w], vwu | fm);
-                    my_found += found;
-                    if (BW && found) gmin = min(gmin, v);
+                    const uint64_t fm = __ballot(hit);
+                    if (fm && c.lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + w), (unsigned long long)fm);
+                    my_found += hit;
+                    if (BW && hit) gmin = min(gmin, v);
                 }
             }
+            scan_misses();
             c.edges_bu += block_sum(scanned, sm);
         }
         // ---- level totals: every block computes the same sums
@@ -479,7 +503,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             for (int64_t wb = gwave * kWave; wb < a.nwords; wb += total_waves * kWave) {
                 const int64_t w = wb + c.lane;
                 uint64_t nw = 0;
-                if (w < a.nwords) nw = load_agent(&vis[w]) & ~load_agent(&a.snap[w]);
+                if (w < a.nwords) nw = vis[w] & ~a.snap[w];
                 uint64_t busy = __ballot(nw != 0);
                 while (busy) {
                     const int l = __ffsll((unsigned long long)busy) - 1;
@@ -515,7 +539,7 @@ __device__ __forceinline__ int64_t trim_sweep(Ctx& c, const int32_t* fin, int64_
         int32_t v = 0;
         bool alive = false;
         if (i < n) {
-            v = load_agent(&fin[i]);
+            v = fin[i];
             alive = bit_of(a.act, v);
         }
         const bool out = wave_any_edge(alive, alive ? a.fwb[v] : 0, alive ? a.fwb[v + 1] : 0, v, a.fwa, live, scanned);
@@ -580,16 +604,16 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
         // P8-P10: pattern detection against the current act, then commit
         const int32_t* f = buf(cur);
         for (int64_t i = gt; i < n; i += nthreads) {
-            const int32_t v = load_agent(&f[i]);
+            const int32_t v = f[i];
             if (bit_of(a.act, v)) pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v);
         }
         if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         int c2 = 0, c3 = 0;
         for (int64_t i = gt; i < n; i += nthreads) {
-            const int32_t v = load_agent(&f[i]);
-            if (load_agent(&a.lab[v]) >= 0 && bit_of(a.act, v)) {
+            const int32_t v = f[i];
+            if (a.lab[v] >= 0 && bit_of(a.act, v)) {
                 atomicAnd(reinterpret_cast<unsigned long long*>(a.act + (v >> 6)), ~(1ull << (v & 63)));
-                if (load_agent(&a.stamp[v]) == 2) ++c2;
+                if (a.stamp[v] == 2) ++c2;
                 else ++c3;
             }
         }
@@ -604,15 +628,15 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
             // P12: WCC by union-find; parent = smallest id of the WCC
             const int32_t* g = buf(cur);
             for (int64_t i = gt; i < n; i += nthreads) {
-                const int32_t v = load_agent(&g[i]);
+                const int32_t v = g[i];
                 st_agent(&a.parent[v], v);
             }
             if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
             for (int64_t i = gt; i < n; i += nthreads) {
-                const int32_t v = load_agent(&g[i]);
+                const int32_t v = g[i];
                 for (int32_t k = a.fwb[v]; k < a.fwb[v + 1]; ++k) {
                     const int32_t w = a.fwa[k];
-                    if (w == v || !((load_agent(&a.act[w >> 6]) >> (w & 63)) & 1ull)) continue;
+                    if (w == v || !bit_of(a.act, w)) continue;
                     int32_t x = v, y = w;
                     while (true) {
                         x = uf_root_g(a.parent, x);
@@ -629,7 +653,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
             }
             if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
             for (int64_t i = gt; i < n; i += nthreads) {
-                const int32_t v = load_agent(&g[i]);
+                const int32_t v = g[i];
                 st_agent(&a.parent[v], uf_root_g(a.parent, v));
             }
             wcc = true;
@@ -667,28 +691,44 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     // cleared traversal bitmaps, pivot = largest out*in (lowest id on ties).
     int64_t bestp = 0, surv = 0;
     int32_t besti = INT32_MAX;
-    for (int64_t w = gwave; w < a.nwords; w += total_waves) {
-        const int64_t v = w * 64 + c.lane;
-        const uint64_t sw = a.sel ? a.sel[w] : ~0ull;  // vertices outside the selection never become active
-        bool s = false;
-        if (v < a.V && ((sw >> c.lane) & 1ull)) {
-            const int32_t od = a.fwb[v + 1] - a.fwb[v];
-            const int32_t id = a.bwb[v + 1] - a.bwb[v];
-            s = od > 0 && id > 0;
+    // a wave takes kUA consecutive words at a time and issues all their offset
+    // loads before it uses any of them
+    constexpr int kUA = 4;
+    for (int64_t w0 = gwave * kUA; w0 < a.nwords; w0 += total_waves * kUA) {
+        int32_t od[kUA], id[kUA];
+        bool on[kUA];
+#pragma unroll
+        for (int u = 0; u < kUA; ++u) {
+            const int64_t w = w0 + u, v = w * 64 + c.lane;
+            // vertices outside the selection never become active
+            const uint64_t sw = w < a.nwords ? (a.sel ? a.sel[w] : ~0ull) : 0ull;
+            on[u] = v < a.V && ((sw >> c.lane) & 1ull);
+            od[u] = 0;
+            id[u] = 0;
+            if (on[u]) {
+                od[u] = a.fwb[v + 1] - a.fwb[v];
+                id[u] = a.bwb[v + 1] - a.bwb[v];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUA; ++u) {
+            const int64_t w = w0 + u, v = w * 64 + c.lane;
+            const bool s = on[u] && od[u] > 0 && id[u] > 0;
             if (s) {
-                const int64_t p = a.wrap32 ? (int64_t)(int32_t)((uint32_t)od * (uint32_t)id) : (int64_t)od * id;
+                const int64_t p =
+                    a.wrap32 ? (int64_t)(int32_t)((uint32_t)od[u] * (uint32_t)id[u]) : (int64_t)od[u] * id[u];
                 if (p > 0 && pivot_better(p, (int32_t)v, bestp, besti)) {
                     bestp = p;
                     besti = (int32_t)v;
                 }
             }
-        }
-        const uint64_t m = __ballot(s);
-        if (c.lane == 0) {
-            st_agent(&a.act[w], (uint64_t)m);
-            st_agent(&a.fvis[w], (uint64_t)0);
-            st_agent(&a.bvis[w], (uint64_t)0);
-            surv += __popcll(m);
+            const uint64_t m = __ballot(s);
+            if (c.lane == 0 && w < a.nwords) {
+                st_agent(&a.act[w], (uint64_t)m);
+                st_agent(&a.fvis[w], (uint64_t)0);
+                st_agent(&a.bvis[w], (uint64_t)0);
+                surv += __popcll(m);
+            }
         }
     }
 #pragma unroll
@@ -773,18 +813,26 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     const int64_t per = (a.nwords + c.G - 1) / c.G;
     const int64_t wb = min((int64_t)c.b * per, a.nwords), we = min(wb + per, a.nwords);
     int64_t cnt = 0;
-    for (int64_t w = wb + c.wave; w < we; w += kW) {
-        const uint64_t a0 = load_agent(&a.act[w]);
-        const uint64_t g = traversed ? load_agent(&a.bvis[w]) : 0ull;
-        const int64_t v = w * 64 + c.lane;
-        const uint64_t sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
-        if (v < a.V && ((sw >> c.lane) & 1ull)) {
-            const bool gi = (g >> c.lane) & 1ull, ac = (a0 >> c.lane) & 1ull;
-            st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (ac ? -1 : (int32_t)v));
-        }
-        if (c.lane == 0) {
+    // one word per lane: the wave loads 64 words at once, then writes their
+    // labels word by word (coalesced 256 B stores, no loads in the loop)
+    for (int64_t base = wb + (int64_t)c.wave * kWave; base < we; base += (int64_t)kW * kWave) {
+        const int64_t w = base + c.lane;
+        uint64_t a0 = 0, g = 0, sw = 0;
+        if (w < we) {
+            a0 = a.act[w];
+            g = traversed ? a.bvis[w] : 0ull;
+            sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
             if (g) st_agent(&a.act[w], a0 & ~g);
             cnt += __popcll(a0 & ~g);
+        }
+        const int nw = (int)min<int64_t>(kWave, we - base);
+        for (int j = 0; j < nw; ++j) {
+            const uint64_t aj = __shfl(a0, j, kWave), gj = __shfl(g, j, kWave), sj = __shfl(sw, j, kWave);
+            const int64_t v = (base + j) * 64 + c.lane;
+            if (v < a.V && ((sj >> c.lane) & 1ull)) {
+                const bool gi = (gj >> c.lane) & 1ull, ac = (aj >> c.lane) & 1ull;
+                st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (ac ? -1 : (int32_t)v));
+            }
         }
     }
     const int64_t cnt_blk = block_sum(cnt, sm);
@@ -810,7 +858,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     }
     for (int64_t base = wb; base < we; base += kGBlock) {
         const int64_t w = base + c.tid;
-        const uint64_t m = w < we ? load_agent(&a.act[w]) : 0ull;
+        const uint64_t m = w < we ? a.act[w] : 0ull;
         int excl, total;
         block_excl_scan<kW>(__popcll(m), excl, total, sm.tmp);
         uint64_t mm = m;
