@@ -413,42 +413,55 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                 nmiss = 0;
                 wave_sync();
             };
-            for (int64_t w0 = gwave * kU; w0 < a.nwords; w0 += total_waves * kU) {
-                // lane u < kU holds word w0 + u; the others read it by shuffle
-                uint64_t my_vw = 0, my_cand = 0;
-                if (c.lane < kU && w0 + c.lane < a.nwords) {
-                    my_vw = vis[w0 + c.lane];
-                    my_cand = allow[w0 + c.lane] & ~my_vw;
-                    st_agent(&a.snap[w0 + c.lane], my_vw);
+            for (int64_t wb = gwave * kWave; wb < a.nwords; wb += total_waves * kWave) {
+                // lane j loads word wb + j; then only words with candidates
+                // are worked on, kU of them at a time
+                const int64_t wl = wb + c.lane;
+                uint64_t lvw = 0, lcand = 0;
+                if (wl < a.nwords) {
+                    lvw = vis[wl];
+                    lcand = allow[wl] & ~lvw;
+                    st_agent(&a.snap[wl], lvw);
                 }
-                int32_t f[kU];
+                uint64_t busy = __ballot(lcand != 0);
+                while (busy) {
+                    int js[kU];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    f[u] = -1;
-                    if ((__shfl(my_cand, u, kWave) >> c.lane) & 1ull) f[u] = rfirst[(w0 + u) * 64 + c.lane];
-                }
-                uint32_t hits = 0;
-#pragma unroll
-                for (int u = 0; u < kU; ++u) hits |= (f[u] >= 0 && bit_of(vis, f[u])) ? 1u << u : 0u;
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    const uint64_t cm = __shfl(my_cand, u, kWave);
-                    if (!cm) continue;
-                    const int64_t w = w0 + u;
-                    const int32_t v = (int32_t)(w * 64 + c.lane);
-                    const bool is_c = (cm >> c.lane) & 1ull;
-                    const bool hit = is_c && ((hits >> u) & 1u);
-                    scanned += is_c;
-                    const uint64_t mm = __ballot(is_c && !hit && f[u] >= 0);  // misses with more list to scan
-                    if (mm) {
-                        if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
-                        if ((mm >> c.lane) & 1ull) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
-                        nmiss += __popcll(mm);
+                    for (int u = 0; u < kU; ++u) {
+                        js[u] = busy ? __ffsll((unsigned long long)busy) - 1 : -1;
+                        busy &= busy ? busy - 1 : 0;
                     }
-                    const uint64_t fm = __ballot(hit);
-                    if (fm && c.lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + w), (unsigned long long)fm);
-                    my_found += hit;
-                    if (BW && hit) gmin = min(gmin, v);
+                    int32_t f[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        f[u] = -1;
+                        if (js[u] >= 0 && ((__shfl(lcand, js[u], kWave) >> c.lane) & 1ull))
+                            f[u] = rfirst[(wb + js[u]) * 64 + c.lane];
+                    }
+                    uint32_t hits = 0;
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) hits |= (f[u] >= 0 && bit_of(vis, f[u])) ? 1u << u : 0u;
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (js[u] < 0) continue;
+                        const uint64_t cm = __shfl(lcand, js[u], kWave);
+                        const int64_t w = wb + js[u];
+                        const int32_t v = (int32_t)(w * 64 + c.lane);
+                        const bool is_c = (cm >> c.lane) & 1ull;
+                        const bool hit = is_c && ((hits >> u) & 1u);
+                        scanned += is_c;
+                        const uint64_t mm = __ballot(is_c && !hit && f[u] >= 0);  // misses with more list to scan
+                        if (mm) {
+                            if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
+                            if ((mm >> c.lane) & 1ull) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
+                            nmiss += __popcll(mm);
+                        }
+                        const uint64_t fm = __ballot(hit);
+                        if (fm && c.lane == 0)
+                            atomicOr(reinterpret_cast<unsigned long long*>(vis + w), (unsigned long long)fm);
+                        my_found += hit;
+                        if (BW && hit) gmin = min(gmin, v);
+                    }
                 }
             }
             scan_misses();
