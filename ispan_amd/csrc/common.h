@@ -13,7 +13,7 @@ constexpr int kTrimTile = 4096;      // vertices per tile in the first trim swee
 constexpr int kHubDeg = 4096;        // top-down: degree at which a vertex is split
 constexpr int kHubChunk = 4096;      // edges per hub work item
 constexpr int kBuTileWords = 32;     // bottom-up: bitmap words (x64 vertices) per tile
-constexpr int kWaveBuf = 256;        // per-wave LDS staging for queue pushes
+constexpr int kWaveBuf = 512;        // per-wave LDS staging for queue pushes
 
 // Device control block: every count a kernel needs to size its work lives
 // here, so kernels launch on fixed grids and the host reads it back only

@@ -39,6 +39,9 @@ struct Smem {
     int32_t bi[kW];
     int ok;
     int32_t wbuf[kW][kWaveBuf];
+    int32_t wcnt[kW];       // block flushes: per-wave counts, offsets, block base
+    int32_t wbase[kW];
+    int32_t bbase;
     int32_t hv[kW][kWave];  // per-wave staged hub vertices: id, first edge, degree
     int32_t hs[kW][kWave];
     int32_t hd[kW][kWave];
@@ -218,6 +221,52 @@ __device__ __forceinline__ void hub_push(HubStage& hs, bool pred, int32_t v, int
     hs.n += c;
 }
 
+// End-of-phase flushes of every wave's staged entries with one allocation
+// per block: each wave's count goes through LDS, thread 0 reserves the
+// block's total.  (One allocation per wave would put up to 4096 atomics on
+// one counter at the end of every level.)  All threads call them.
+__device__ __forceinline__ int block_reserve(Ctx& c, int wave_count, int32_t* gcount) {
+    Smem& sm = c.sm;
+    if (c.lane == 0) sm.wcnt[c.wave] = wave_count;
+    __syncthreads();
+    if (c.tid == 0) {
+        int tot = 0;
+        for (int w = 0; w < kW; ++w) {
+            sm.wbase[w] = tot;
+            tot += sm.wcnt[w];
+        }
+        sm.bbase = tot ? atomicAdd(gcount, tot) : 0;
+    }
+    __syncthreads();
+    const int base = sm.bbase + sm.wbase[c.wave];
+    __syncthreads();  // sm.* free again
+    return base;
+}
+
+__device__ __forceinline__ void block_flush(Ctx& c, WaveQueue& wq, int32_t* gq, int32_t* gcount) {
+    const int base = block_reserve(c, wq.n, gcount);
+    for (int i = c.lane; i < wq.n; i += kWave) st_agent(&gq[base + i], wq.buf[i]);
+    wq.n = 0;
+    wave_sync();
+}
+
+__device__ __forceinline__ void block_hub_flush(Ctx& c, HubStage& hs, HubItem* hout, int32_t* hn_out) {
+    int32_t v = 0, s0 = 0, nch = 0;
+    if (c.lane < hs.n) {
+        v = hs.v[c.lane];
+        s0 = hs.s[c.lane];
+        nch = (hs.d[c.lane] + kGiantChunk - 1) / kGiantChunk;
+    }
+    const int incl = wave_incl_scan(nch);
+    const int base = block_reserve(c, __shfl(incl, kWave - 1, kWave), hn_out) + incl - nch;
+    for (int k = 0; k < nch; ++k) {
+        st_agent(&hout[base + k].v, v);
+        st_agent(&hout[base + k].start, s0 + k * kGiantChunk);
+    }
+    hs.n = 0;
+    wave_sync();
+}
+
 // Pushes a newly claimed vertex for expansion at the next level: vertices of
 // degree >= kGiantHubDeg become kGiantChunk-edge work items, others go to
 // the queue (degree-0 vertices have nothing to expand).  All lanes call it.
@@ -278,8 +327,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             seeded += ok;
             push_next(ok, r, beg, wq, a.q0, &gs->qn[dir][0].v, hs, a.h0, &gs->hn[dir][0].v, fedges);
         }
-        wq_flush<true>(wq, a.q0, &gs->qn[dir][0].v);
-        hub_flush(hs, a.h0, &gs->hn[dir][0].v);
+        block_flush(c, wq, a.q0, &gs->qn[dir][0].v);
+        block_hub_flush(c, hs, a.h0, &gs->hn[dir][0].v);
         Red r;
         if (!gsync<true>(gs, c.G, sm, a.dbg, Red{block_sum(seeded, sm), 0, 0, 0}, &r)) return -1;
         *seeded_out = r.s0;
@@ -379,8 +428,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     }
                 }
             }
-            wq_flush<true>(wq, qout, qn_out);
-            hub_flush(hs, hout, hn_out);
+            block_flush(c, wq, qout, qn_out);
+            block_hub_flush(c, hs, hout, hn_out);
         } else {
             // ---- bottom-up over the candidate bitmap, one wave per group of
             // kU consecutive 64-vertex words; only that wave writes those
@@ -526,8 +575,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next, fedges);
                 }
             }
-            wq_flush<true>(wq, q_next, qn_next);
-            hub_flush(hs, h_next, hn_next);
+            block_flush(c, wq, q_next, qn_next);
+            block_hub_flush(c, hs, h_next, hn_next);
             mode = 2;
             if (!gsync(gs, c.G, sm, a.dbg)) return -1;
         }
@@ -566,7 +615,7 @@ __device__ __forceinline__ int64_t trim_sweep(Ctx& c, const int32_t* fin, int64_
         }
         wq_push<true>(wq, alive && !trim, v, fout, fcount);
     }
-    wq_flush<true>(wq, fout, fcount);
+    block_flush(c, wq, fout, fcount);
     return block_sum(trimmed, c.sm);
 }
 
