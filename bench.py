@@ -292,6 +292,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--timing", type=int, default=2, help="library timing level inside the timed region")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuned measurement beside the main line")
     ap.add_argument("--batch-graphs", type=int, default=4096,
                     help="batched leg: number of graphs (0 skips it)")
     ap.add_argument("--served-requests", type=int, default=1000,
@@ -360,6 +362,23 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": dict(cfg, V=V, E=E),
     }
+    if a.autotune:
+        # the project's autotuner: fastest (alpha, beta) on this graph; the
+        # labels cannot change, and the main line above keeps the defaults
+        cands = I.tuning_grid(alphas=(2, 4, 10, 30), betas=(50, 200, 800))
+        best, cms = S.autotune(cands, reps=3)
+        S.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            S.solve(best)
+        S.synchronize()
+        tuned_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+        same = bool(np.array_equal(S.labels(), labels))
+        result["autotuned"] = {"alpha": best.alpha, "beta": best.beta, "ms_per_step": tuned_ms,
+                               "value": E / (tuned_ms * 1e-3), "labels_equal_default": same,
+                               "candidates": [{"alpha": c.alpha, "beta": c.beta, "median_ms": m}
+                                              for c, m in zip(cands, cms)]}
+        log(f"[tune] alpha={best.alpha} beta={best.beta}: {tuned_ms:.3f} ms per solve (default {ms_per_step:.3f})")
     # BFS phases against the HBM roofline (algorithmic bytes from the final sets)
     fw_ms, bw_ms = st_phase.ms[7], st_phase.ms[8]
     bfs_bytes = 4.0 * (st_phase.fw_edges + st_phase.bw_edges) + 12.0 * (st_phase.fw_size + st_phase.giant_size)

@@ -189,6 +189,14 @@ int ispan_batch_solve(ispan_ctx* ctx, const ispan_params* params, ispan_stats* s
 int ispan_batch_labels_download(ispan_ctx* ctx, int32_t* labels);
 const int32_t* ispan_batch_labels_device(const ispan_ctx* ctx);
 
+/* Autotuning: times every candidate parameter set on the bound graph (or
+ * the bound batch) `reps` times after one warm-up solve and reports the
+ * fastest (its index in *best, median ms of each in ms[n] if not NULL).
+ * Parameters choose the work, never the labels, so any candidate is valid.
+ * No counterpart in the reference, whose alpha/beta/gamma come from argv. */
+int ispan_autotune(ispan_ctx* ctx, const ispan_params* candidates, int32_t n, int32_t reps, int32_t* best,
+                   double* ms);
+
 /* ---- serving --------------------------------------------------------------
  * A request loop over device-resident graphs with continuous batching; no
  * counterpart in the reference.  Registered graphs are appended once to an

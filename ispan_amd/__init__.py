@@ -37,7 +37,7 @@ EXPORTS = [
     "ispan_labels_device", "ispan_solve_stats", "ispan_scc_detection", "ispan_graph_load", "ispan_graph_free",
     "ispan_csr_check", "ispan_summarize", "ispan_gen_rmat_device", "ispan_gen_grid_device",
     "ispan_build_csr_device", "ispan_device_free", "ispan_version", "ispan_kernel_profile_get",
-    "ispan_convert_edge_list", "ispan_batch_bind_device", "ispan_batch_groups", "ispan_batch_solve",
+    "ispan_convert_edge_list", "ispan_autotune", "ispan_batch_bind_device", "ispan_batch_groups", "ispan_batch_solve",
     "ispan_batch_labels_download", "ispan_batch_labels_device", "ispan_server_create", "ispan_server_destroy",
     "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit", "ispan_server_submit_many",
     "ispan_server_wait", "ispan_server_wait_device", "ispan_server_release", "ispan_server_stats_get",
@@ -189,6 +189,7 @@ def lib():
         "ispan_batch_bind_device": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int64,
                                               C.POINTER(Params)]),
         "ispan_batch_groups": (C.c_int, [_vp]),
+        "ispan_autotune": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _vp]),
         "ispan_batch_solve": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Stats)]),
         "ispan_batch_labels_download": (C.c_int, [_vp, _vp]),
         "ispan_batch_labels_device": (_vp, [_vp]),
@@ -407,6 +408,14 @@ class Solver:
         _check(lib().ispan_batch_labels_download(self._ctx, _ptr(out)), self._ctx)
         return [out[v[g]:v[g + 1]] for g in range(len(v) - 1)]
 
+    def autotune(self, candidates: list[Params], reps: int = 3) -> tuple[Params, list[float]]:
+        """Time each candidate on the bound graph or batch; -> (fastest, median ms of each)."""
+        arr = (Params * len(candidates))(*candidates)
+        ms = np.zeros(len(candidates), np.float64)
+        best = C.c_int32()
+        _check(lib().ispan_autotune(self._ctx, arr, len(candidates), reps, C.byref(best), _ptr(ms)), self._ctx)
+        return candidates[best.value], ms.tolist()
+
     def gen_rmat(self, V: int, edge_factor: float = 16.0, seed: int = 1, a=0.57, b=0.19, c=0.19) -> DeviceGraph:
         ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]
         E = C.c_int64()
@@ -591,6 +600,11 @@ class DeviceBatch:
             if p:
                 hiprt.free(p)
         self.fw_beg = self.fw_adj = self.bw_beg = self.bw_adj = 0
+
+
+def tuning_grid(alphas=(4, 10, 30, 100), betas=(50, 200, 800), **kw) -> list[Params]:
+    """Candidate parameter sets for Solver.autotune: every (alpha, beta) pair."""
+    return [default_params(alpha=a, beta=b, **kw) for a in alphas for b in betas]
 
 
 def scc_detection(g: HostGraph, alpha: int = 30, beta: int = 200, gamma: int = 10, theta: float = 0.1,

@@ -1315,6 +1315,36 @@ int ispan_batch_solve(ispan_ctx* ctx, const ispan_params* params, ispan_stats* s
     return ISPAN_OK;
 }
 
+int ispan_autotune(ispan_ctx* ctx, const ispan_params* candidates, int32_t n, int32_t reps, int32_t* best,
+                   double* ms) {
+    if (!ctx || !candidates || n < 1 || reps < 1 || !best) return ISPAN_ERR_ARG;
+    const bool batch = ctx->batch_n >= 0;
+    if (!batch && ctx->V < 0) return ISPAN_ERR_NOGRAPH;
+    CK(hipSetDevice(ctx->device));
+    double best_ms = 0.0;
+    *best = -1;
+    for (int32_t i = 0; i < n; ++i) {
+        const ispan_params& P = candidates[i];
+        if (P.alpha <= 0 || P.beta <= 0 || P.trim_times < 0) return ISPAN_ERR_ARG;
+        std::vector<double> t;
+        for (int32_t r = 0; r <= reps; ++r) {
+            const double t0 = wall_ms();
+            const int rc = batch ? ispan_batch_solve(ctx, &P, nullptr) : solve_impl(ctx, &P, nullptr, nullptr);
+            if (rc) return rc;
+            CK(hipStreamSynchronize(ctx->stream));
+            if (r > 0) t.push_back(wall_ms() - t0);  // the first run warms up
+        }
+        std::sort(t.begin(), t.end());
+        const double med = t[t.size() / 2];
+        if (ms) ms[i] = med;
+        if (*best < 0 || med < best_ms) {
+            *best = i;
+            best_ms = med;
+        }
+    }
+    return ISPAN_OK;
+}
+
 int ispan_batch_labels_download(ispan_ctx* ctx, int32_t* labels) {
     if (!ctx || !labels) return ISPAN_ERR_ARG;
     if (!ctx->batch_solved) return ISPAN_ERR_NOGRAPH;

@@ -362,3 +362,15 @@ def test_engines_structured(solver, engine):
     for name, d in G.pattern_cases():
         lab, _st = solve(solver, G.pattern_graph(d), engine=engine)
         assert lab.tolist() == d["canonical"], name
+
+
+def test_autotune_picks_a_candidate_and_labels_do_not_change(solver):
+    g = O.grid(64, 0.6, 3)
+    solver.upload(I.HostGraph(g.fw_beg, g.fw_adj, g.bw_beg, g.bw_adj))
+    cands = I.tuning_grid(alphas=(2, 30), betas=(50, 200))
+    best, ms = solver.autotune(cands, reps=2)
+    assert best in cands and len(ms) == 4 and min(ms) > 0
+    ref = O.tarjan(g)
+    for p in cands:
+        solver.solve(p)
+        assert np.array_equal(solver.labels(), ref)
