@@ -88,6 +88,8 @@ struct GiantState {
     // per level of each direction (first kLevelTrace levels): end time,
     // vertices found, adjacency entries read, 0 top-down / 1 bottom-up
     int64_t lv_ts[2][64], lv_found[2][64], lv_edges[2][64], lv_mode[2][64];
+    // blocks' start and first-barrier arrival clocks: min, max of each
+    int64_t t_start[2], t_arrive[2];
 };
 constexpr int kLevelTrace = 64;
 

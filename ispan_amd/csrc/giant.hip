@@ -748,49 +748,65 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     const int64_t gwave = (int64_t)c.b * kW + c.wave;
     const bool stamp = c.b == 0 && c.tid == 0;  // phase timestamps (constant-rate clock)
     if (stamp) gs->ts[0] = wall_clock64();
+    if (c.tid == 0) {  // launch ramp: when each block started
+        const unsigned long long t = wall_clock64();
+        atomicMax(reinterpret_cast<unsigned long long*>(&gs->t_start[0]), ~t);  // min as max of ~t (state starts at 0)
+        atomicMax(reinterpret_cast<unsigned long long*>(&gs->t_start[1]), t);
+    }
 
     // ---- phase A: trim-1-first (raw degree 0 either way), act bitmap,
     // cleared traversal bitmaps, pivot = largest out*in (lowest id on ties).
     int64_t bestp = 0, surv = 0;
     int32_t besti = INT32_MAX;
     // a wave takes kUA consecutive words at a time and issues all their offset
-    // loads before it uses any of them
-    constexpr int kUA = 4;
+    // loads before it uses any of them; lane i loads beg[v] once and takes
+    // beg[v + 1] from lane i + 1 (lane 63 from the next word's lane 0)
+    constexpr int kUA = 8;
     for (int64_t w0 = gwave * kUA; w0 < a.nwords; w0 += total_waves * kUA) {
-        int32_t od[kUA], id[kUA];
-        bool on[kUA];
+        int32_t fo[kUA], bo[kUA];
+#pragma unroll
+        for (int u = 0; u < kUA; ++u) {
+            const int64_t v = (w0 + u) * 64 + c.lane;
+            fo[u] = v <= a.V ? a.fwb[v] : 0;
+            bo[u] = v <= a.V ? a.bwb[v] : 0;
+        }
+        int32_t fl = 0, bl = 0;  // offsets of the vertex after the last word
+        const int64_t vl = (w0 + kUA) * 64;
+        if (c.lane == kWave - 1 && vl <= a.V) {
+            fl = a.fwb[vl];
+            bl = a.bwb[vl];
+        }
+        uint64_t mine = 0;  // lane u < kUA keeps word w0 + u's survivors
 #pragma unroll
         for (int u = 0; u < kUA; ++u) {
             const int64_t w = w0 + u, v = w * 64 + c.lane;
+            const int32_t f0 = u + 1 < kUA ? __shfl(fo[u + 1 < kUA ? u + 1 : u], 0, kWave) : fl;
+            const int32_t b0 = u + 1 < kUA ? __shfl(bo[u + 1 < kUA ? u + 1 : u], 0, kWave) : bl;
+            int32_t fn = __shfl_down(fo[u], 1, kWave), bn = __shfl_down(bo[u], 1, kWave);
+            if (c.lane == kWave - 1) {
+                fn = f0;
+                bn = b0;
+            }
             // vertices outside the selection never become active
             const uint64_t sw = w < a.nwords ? (a.sel ? a.sel[w] : ~0ull) : 0ull;
-            on[u] = v < a.V && ((sw >> c.lane) & 1ull);
-            od[u] = 0;
-            id[u] = 0;
-            if (on[u]) {
-                od[u] = a.fwb[v + 1] - a.fwb[v];
-                id[u] = a.bwb[v + 1] - a.bwb[v];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kUA; ++u) {
-            const int64_t w = w0 + u, v = w * 64 + c.lane;
-            const bool s = on[u] && od[u] > 0 && id[u] > 0;
+            const bool on = v < a.V && ((sw >> c.lane) & 1ull);
+            const int32_t od = fn - fo[u], id = bn - bo[u];
+            const bool s = on && od > 0 && id > 0;
             if (s) {
-                const int64_t p =
-                    a.wrap32 ? (int64_t)(int32_t)((uint32_t)od[u] * (uint32_t)id[u]) : (int64_t)od[u] * id[u];
+                const int64_t p = a.wrap32 ? (int64_t)(int32_t)((uint32_t)od * (uint32_t)id) : (int64_t)od * id;
                 if (p > 0 && pivot_better(p, (int32_t)v, bestp, besti)) {
                     bestp = p;
                     besti = (int32_t)v;
                 }
             }
             const uint64_t m = __ballot(s);
-            if (c.lane == 0 && w < a.nwords) {
-                st_agent(&a.act[w], (uint64_t)m);
-                st_agent(&a.fvis[w], (uint64_t)0);
-                st_agent(&a.bvis[w], (uint64_t)0);
-                surv += __popcll(m);
-            }
+            if (c.lane == u) mine = m;
+        }
+        if (c.lane < kUA && w0 + c.lane < a.nwords) {
+            st_agent(&a.act[w0 + c.lane], mine);
+            st_agent(&a.fvis[w0 + c.lane], (uint64_t)0);
+            st_agent(&a.bvis[w0 + c.lane], (uint64_t)0);
+            surv += __popcll(mine);
         }
     }
 #pragma unroll
@@ -816,6 +832,9 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         st_agent(&a.part[(int64_t)kPivP * c.G + c.b], (int64_t)(bestp));
         st_agent(&a.part[(int64_t)kPivI * c.G + c.b], (int64_t)(besti));
         st_agent(&a.part[(int64_t)kSurv * c.G + c.b], (int64_t)(surv_blk));
+        const unsigned long long t = wall_clock64();  // trim-1-first done in this block
+        atomicMax(reinterpret_cast<unsigned long long*>(&gs->t_arrive[0]), ~t);
+        atomicMax(reinterpret_cast<unsigned long long*>(&gs->t_arrive[1]), t);
     }
     if (!gsync(gs, c.G, sm, a.dbg)) return;
     // every block reduces the pivot candidates and the survivor count

@@ -525,6 +525,12 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                 giant_phase_ms[3] = gs.ts[5] > gs.ts[4] ? (gs.ts[5] - gs.ts[4]) * k : 0.0;
             }
             if (ctx->level_trace && ctx->clock_khz > 0) {
+                const double k0 = 1.0 / ctx->clock_khz;
+                fprintf(stderr,
+                        "[ispan] blocks started %.3f..%.3f ms, trim-1-first done %.3f..%.3f ms, pivot known %.3f ms\n",
+                        ((int64_t)~gs.t_start[0] - gs.ts[0]) * k0, (gs.t_start[1] - gs.ts[0]) * k0,
+                        ((int64_t)~gs.t_arrive[0] - gs.ts[0]) * k0, (gs.t_arrive[1] - gs.ts[0]) * k0,
+                        (gs.ts[1] - gs.ts[0]) * k0);
                 const double k = 1.0 / ctx->clock_khz;
                 for (int d = 0; d < 2; ++d) {
                     int64_t t = d == 0 ? gs.ts[1] : gs.ts[2];
