@@ -145,7 +145,9 @@ constexpr int kGiantChunk = 2048;   // edges per work item
 inline int64_t giant_hub_capacity(int64_t E) { return 5 * (E / kGiantChunk) + 64; }
 
 int giant_max_blocks_per_cu();
-hipError_t launch_giant(const GiantArgs& a, int grid, hipStream_t s);
+// zeroes a.gs and ctl, records `before` (when given), then launches k_giant
+// cooperatively
+hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before);
 // batched solves (batch.hip)
 hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, const int32_t* bwb_cat,
                               const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,

@@ -996,11 +996,20 @@ int giant_max_blocks_per_cu() {
     return nb;
 }
 
-hipError_t launch_giant(const GiantArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<int64_t*>(a.gs),
-                       (int64_t)(sizeof(GiantState) / 8));
+// Clears the cooperative kernel's state and the solve's control block in one
+// launch (both are a few KB).
+__global__ void k_zero_state(int64_t* gs, int64_t ngs, int64_t* ctl, int64_t nctl) {
+    for (int64_t i = threadIdx.x; i < ngs; i += blockDim.x) gs[i] = 0;
+    for (int64_t i = threadIdx.x; i < nctl; i += blockDim.x) ctl[i] = 0;
+}
+
+hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before) {
+    static_assert(sizeof(GiantState) % 8 == 0 && sizeof(Ctl) % 8 == 0, "state sizes");
+    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(1024), 0, s, reinterpret_cast<int64_t*>(a.gs),
+                       (int64_t)(sizeof(GiantState) / 8), reinterpret_cast<int64_t*>(ctl), (int64_t)(sizeof(Ctl) / 8));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (before && (e = hipEventRecord(before, s)) != hipSuccess) return e;
     GiantArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args, 0, s);

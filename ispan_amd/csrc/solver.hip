@@ -258,7 +258,9 @@ int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
 hipEvent_t pool_event(ispan_ctx* ctx) {
     if (ctx->pool_used == ctx->pool.size()) {
         hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        // timing only, read after a stream synchronize: no system-scope fence
+        // (it would write the L2s back to memory at every record)
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
         ctx->pool.push_back(e);
     }
     return ctx->pool[ctx->pool_used++];
@@ -406,8 +408,14 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     ctx->recs.clear();
     ctx->pool_used = 0;
     ctx->prof = ispan_kernel_profile{};
-    CK(launch_reset_ctl(ctx->ctl, s));
-    rec(ctx, P, 0);
+    // Engine 0 needs a cooperative launch (all blocks resident at once).  The
+    // device grants it to one process at a time; when it is refused (or the
+    // device has none) this solve runs engine 1, and stats.engine_used says so.
+    bool coop = P->engine == 0 && ctx->coop_grid > 0 && V > 0;
+    if (!coop) CK(launch_reset_ctl(ctx->ctl, s));  // the cooperative launch clears it with its own state
+    // Phase events bracket the host-driven phases; the cooperative kernel
+    // times its own phases with the device clock, so it records none.
+    if (!coop) rec(ctx, P, 0);
     if (V == 0) {
         CK(hipStreamSynchronize(s));
         ctx->have_solution = true;
@@ -418,10 +426,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     int32_t root = -1;
     bool rest_done = false;
     double giant_phase_ms[4] = {0, 0, 0, 0};  // engine 0: trim-1-first+pivot, FW, BW+relabel, trims+WCC
-    // Engine 0 needs a cooperative launch (all blocks resident at once).  The
-    // device grants it to one process at a time; when it is refused (or the
-    // device has none) this solve runs engine 1, and stats.engine_used says so.
-    bool coop = P->engine == 0 && ctx->coop_grid > 0;
+    bool counted = false;  // ctl->unlabelled holds the count of the finished labelling
     if (coop) {
         // P1-P6 + relabel in one cooperative kernel
         GiantArgs ga{};
@@ -465,24 +470,21 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.fq_n_ctl = ctx->ctl->fq_n;
         ga.dbg = ctx->gdbg;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
-        if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) {
-            ctx->recs.push_back({3, ga0, ga1});
-            CK(hipEventRecord(ga0, s));
-        }
-        const hipError_t le = launch_giant(ga, ctx->coop_grid, s);
+        if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) ctx->recs.push_back({3, ga0, ga1});
+        const hipError_t le = launch_giant(ga, ctx->ctl, ctx->coop_grid, s, ga0);
         if (le == hipErrorCooperativeLaunchTooLarge || le == hipErrorIllegalState || le == hipErrorNotSupported ||
             le == hipErrorLaunchOutOfResources) {
             (void)hipGetLastError();
             if (ga0) ctx->recs.pop_back();
             coop = false;
+            rec(ctx, P, 0);  // engine 1 times its phases with events
         } else {
             CK(le);
             if (ga1) CK(hipEventRecord(ga1, s));
-            rec(ctx, P, 1);
-            rec(ctx, P, 2);
-            rec(ctx, P, 3);
-            rec(ctx, P, 4);
             CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, sizeof(GiantState), hipMemcpyDeviceToHost, s));
+            // the kernel normally labels every vertex: count the unlabelled
+            // ones now, so that one host wait covers the solve
+            CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
             if ((rc = readback(ctx, &syncs))) return rc;
             const GiantState& gs = *ctx->h_gstate;
             if (gs.err.v) {
@@ -544,6 +546,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                 }
             }
             rest_done = true;
+            counted = gs.n_final == 0;
             ctx->prof.edges[0] = gs.work[0];
             ctx->prof.edges[2] = gs.work[1];
             ctx->prof.found[0] = gs.work[2];
@@ -605,9 +608,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         pat2 = gs.pat2;
         pat3 = gs.pat3;
         wcc_ready = gs.wcc_done != 0;
-        rec(ctx, P, 5);
-        rec(ctx, P, 6);
-        rec(ctx, P, 7);
+        if (n > 0) rec(ctx, P, 7);  // host-side WCC and mice follow
     } else {
         // P7 trim-1 sweeps with the theta stop rule (src/scc_core.cpp:562-602):
         // the first comparison is against the pre-giant queue, as there.
@@ -648,6 +649,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             rec(ctx, P, 7);
         }
     }
+    const bool host_tail = n > 0;
     if (n > 0) {
         // P12 WCC of the remaining subgraph
         if (P->wcc_mode == 1) {
@@ -678,12 +680,15 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         CK(launch_mice(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->keys_alt, (int32_t)n,
                        ctx->starts, ctx->ctl, ctx->qbuf, G, s));
         local.mice = n;
-    } else {
+    } else if (!coop) {
         rec(ctx, P, 8);
     }
-    rec(ctx, P, 9);
-    CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
-    if ((rc = readback(ctx, &syncs))) return rc;
+    if (!coop || host_tail) rec(ctx, P, 9);
+    if (!counted) {
+        CK(hipMemsetAsync(&ctx->ctl->unlabelled, 0, sizeof(ctx->ctl->unlabelled), s));
+        CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
+        if ((rc = readback(ctx, &syncs))) return rc;
+    }
     if (ctx->h_ctl->unlabelled) {
         ctx->err = std::to_string(ctx->h_ctl->unlabelled) + " vertices left unlabelled";
         return ISPAN_ERR_UNLABELLED;
@@ -703,16 +708,24 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         // event map: 0 start, 1 trim-1-first+queue+pivot, 2 pivot readback,
         // 3 FW done, 4 BW + giant relabel done, 5 trim-1 sweeps, 6 size-2/3,
         // 7 last trim-1 sweep, 8 WCC, 9 mice
-        double first = ev_ms(ctx, 0, 1), fw = ev_ms(ctx, 2, 3), bw = ev_ms(ctx, 3, 4);
-        if (P->engine == 0) {
+        double first, fw, bw, rest_ms = 0.0, t1 = 0.0, t23 = 0.0, t1b = 0.0, wcc = 0.0, mice = 0.0;
+        if (coop) {
             first = giant_phase_ms[0];
             fw = giant_phase_ms[1];
             bw = giant_phase_ms[2];
+            rest_ms = giant_phase_ms[3];
+        } else {
+            first = ev_ms(ctx, 0, 1);
+            fw = ev_ms(ctx, 2, 3);
+            bw = ev_ms(ctx, 3, 4);
+            t1 = ev_ms(ctx, 4, 5);
+            t23 = ev_ms(ctx, 5, 6);
+            t1b = ev_ms(ctx, 6, 7);
         }
-        const double rest_ms = P->engine == 0 ? giant_phase_ms[3] : 0.0;
-        const double
-                     t1 = ev_ms(ctx, 4, 5), t23 = ev_ms(ctx, 5, 6), t1b = ev_ms(ctx, 6, 7), wcc = ev_ms(ctx, 7, 8),
-                     mice = ev_ms(ctx, 8, 9);
+        if (!coop || host_tail) {
+            wcc = ev_ms(ctx, 7, 8);
+            mice = ev_ms(ctx, 8, 9);
+        }
         local.ms[11] = first;
         local.ms[14] = t1 + t1b + rest_ms;  // engine 0: in-kernel sweeps, patterns and WCC
         local.ms[4] = first + t1 + t1b + rest_ms;
@@ -735,7 +748,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             pf.launches[r.cls] += 1;
             pf.ms[r.cls] += ms;
         }
-        if (P->engine != 0) {
+        if (!coop) {
             for (int c = 0; c < 3; ++c) {
                 pf.edges[c] = ctx->h_ctl->work_edges[c];
                 pf.found[c] = ctx->h_ctl->work_found[c];
@@ -851,8 +864,8 @@ int ispan_ctx_create(ispan_ctx** out, int device) {
         return fail(ISPAN_ERR_NOMEM);
     if (hipMalloc(&ctx->ctl, sizeof(Ctl)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
     if (hipHostMalloc(&ctx->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
-    for (auto& e : ctx->ev)
-        if (hipEventCreate(&e) != hipSuccess) return fail(ISPAN_ERR_HIP);
+    for (auto& e : ctx->ev)  // phase timing events, read after a stream synchronize
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return fail(ISPAN_ERR_HIP);
     *out = ctx;
     return ISPAN_OK;
 }
