@@ -1,6 +1,7 @@
 // Internal declarations shared by the kernels and the host-side solver.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <string>
@@ -69,6 +70,7 @@ struct alignas(128) RLine {
 };
 
 // Device state of one k_giant launch; zeroed before every launch.
+constexpr int kLevelTrace = 1024;
 struct GiantState {
     GLine sub[32];      // barrier arrival counters (two-level barrier)
     GLine top;
@@ -85,13 +87,15 @@ struct GiantState {
     RLine gacc[32];     // barrier reductions: per arrival group
     RLine tacc;         // barrier reductions: over groups
     RLine res[2];       // barrier reductions: result of generation g in res[g & 1]
-    // per level of each direction (first kLevelTrace levels): end time,
-    // vertices found, adjacency entries read, 0 top-down / 1 bottom-up
-    int64_t lv_ts[2][64], lv_found[2][64], lv_edges[2][64], lv_mode[2][64];
     // blocks' start and first-barrier arrival clocks: min, max of each
     int64_t t_start[2], t_arrive[2];
+    // per level of each direction (first kLevelTrace levels): end time,
+    // vertices found, adjacency entries read, 0 top-down / 1 bottom-up.
+    // Last on purpose: a solve clears and copies back only what precedes
+    // them, unless the level trace is on.
+    int64_t lv_ts[2][kLevelTrace], lv_found[2][kLevelTrace], lv_edges[2][kLevelTrace], lv_mode[2][kLevelTrace];
 };
-constexpr int kLevelTrace = 64;
+constexpr size_t kGiantStateHead = offsetof(GiantState, lv_ts);
 
 struct GiantArgs {
     const int32_t* fwb;

@@ -29,9 +29,10 @@ constexpr int kW = kGBlock / kWave;
 enum Slot { kPivP, kPivI, kSurv, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
 static_assert(kFoundBU < kGiantSlots, "slot count");
 
+constexpr int kTileMax = kGBlock * 4;  // queue entries per top-down tile, at most
 struct Smem {
-    int32_t off[kGBlock];
-    int32_t start[kGBlock];
+    int32_t off[kTileMax];
+    int32_t start[kTileMax];
     int tmp[kW];
     int64_t red[kW];
     int64_t red2[kW];
@@ -75,8 +76,13 @@ struct Red {
 // the group's last arriver, the grand total published by the last block
 // before it bumps `gen`; every block gets the same `out`.
 //
+// ACQ: after the wait, invalidate this CU's L1 and this XCC's L2 (agent
+// acquire), so that plain loads see what other blocks wrote before the
+// barrier.  Without it only coherent loads and atomics may read such data
+// (and loads that tolerate stale values).
+//
 // Returns false (and flags an error) if a wait exceeds kSpinCap.
-template <bool RED = false>
+template <bool RED = false, bool ACQ = true>
 __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg, Red in = Red{0, 0, 0, 0},
                                       Red* out = nullptr) {
     using ull = unsigned long long;
@@ -136,7 +142,8 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // this CU's L1 and this XCC's L2
+        if (ACQ) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // this CU's L1 and this XCC's L2
+        else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (RED && ok) {
             const RLine& r = gs->res[(g0 + 1) & 1];
             sm.rres[0] = load_agent(&r.sum[0]);
@@ -283,6 +290,101 @@ __device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* 
     wq_push<true>(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
 }
 
+// The acquire gsync<.., true> performs, for a block that passed a barrier
+// without it.
+__device__ __forceinline__ void block_acquire() {
+    if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+}
+
+// Claims K targets per lane, each as wave_try_claim does, and loads the
+// offsets of every candidate while the claim atomics are in flight: a
+// claimed vertex's degree costs no round trip of its own.  All lanes call it.
+template <int K>
+__device__ __forceinline__ void claim_with_degree(const bool (&want)[K], const int32_t (&w)[K],
+                                                  const uint64_t* __restrict__ allow, uint64_t* vis,
+                                                  const int32_t* __restrict__ beg, bool (&claim)[K],
+                                                  int32_t (&s0)[K], int32_t (&d)[K]) {
+    const int lane = lane_id();
+    uint64_t al[K], vi[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        al[k] = 0;
+        vi[k] = 0;
+        if (want[k]) {
+            al[k] = allow[w[k] >> 6];
+            vi[k] = vis[w[k] >> 6];
+        }
+    }
+    bool cand[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) cand[k] = want[k] && ((al[k] & ~vi[k]) >> (w[k] & 63)) & 1ull;
+    uint64_t incl[K], excl[K];
+    int end[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int32_t key = cand[k] ? (w[k] >> 6) : -1 - lane;  // lanes without a candidate: runs of their own
+        const int32_t pkey = __shfl_up(key, 1, kWave);
+        const uint64_t heads = __ballot(lane == 0 || pkey != key);
+        const uint64_t le = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
+        const int start = 63 - __clzll(heads & le);
+        const uint64_t after = heads & ~le;
+        end[k] = after ? __ffsll((unsigned long long)after) - 2 : kWave - 1;
+        uint64_t in = cand[k] ? 1ull << (w[k] & 63) : 0ull;
+#pragma unroll
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+            const uint64_t y = __shfl_up(in, dd, kWave);
+            if (lane - dd >= start) in |= y;
+        }
+        uint64_t ex = __shfl_up(in, 1, kWave);
+        if (lane == start) ex = 0;
+        incl[k] = in;
+        excl[k] = ex;
+    }
+    unsigned long long old[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        old[k] = 0;
+        if (lane == end[k] && incl[k])
+            old[k] = atomicOr(reinterpret_cast<unsigned long long*>(vis + (w[k] >> 6)), (unsigned long long)incl[k]);
+    }
+    int32_t e1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        s0[k] = 0;
+        e1[k] = 0;
+        if (cand[k]) {
+            s0[k] = beg[w[k]];
+            e1[k] = beg[w[k] + 1];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t o = __shfl(old[k], end[k], kWave);
+        claim[k] = cand[k] && !((o | excl[k]) & (1ull << (w[k] & 63)));
+        d[k] = claim[k] ? e1[k] - s0[k] : 0;
+    }
+}
+
+// push_next for K claimed targets whose offsets are known.
+template <int K>
+__device__ __forceinline__ void push_found_n(const bool (&claim)[K], const int32_t (&w)[K], const int32_t (&s0)[K],
+                                             const int32_t (&d)[K], WaveQueue& wq, int32_t* qout, int32_t* qn_out,
+                                             HubStage& hs, HubItem* hout, int32_t* hn_out, int64_t& pushed_edges) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        pushed_edges += d[k];
+        hub_push(hs, claim[k] && d[k] >= kGiantHubDeg, w[k], s0[k], d[k], hout, hn_out);
+        wq_push<true>(wq, claim[k] && d[k] > 0 && d[k] < kGiantHubDeg, w[k], qout, qn_out);
+    }
+}
+
+// Top-down: queue entries per thread per tile, and edges per lane per step
+// (their load chains overlap).
+constexpr int kTE = 1;
+constexpr int kTK = 2;
+constexpr int kTile = kGBlock * kTE;
+
 // One direction of the traversal.  Returns the number of vertices claimed
 // (the roots excluded), or -1 after a failed barrier.  BW: claims are limited
 // to the forward set and *gmin receives the smallest id reached.  With
@@ -366,40 +468,74 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     nh = 0;
                 }
             }
-            const int64_t ntiles = (n + kGBlock - 1) / kGBlock;
+            static_assert(kTile <= kTileMax && (kTile & (kTile - 1)) == 0, "tile size");
+            const int64_t ntiles = (n + kTile - 1) / kTile;
             for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
                 if (u < ntiles) {
-                    const int64_t i = u * kGBlock + c.tid;
-                    int32_t start = 0, deg = 0;
-                    if (i < n) {
-                        const int32_t v = qin[i];
-                        start = beg[v];
-                        deg = beg[v + 1] - start;
+                    // thread tid owns the tile's entries tid * kTE + t
+                    const int64_t i0 = u * kTile + (int64_t)c.tid * kTE;
+                    int32_t qv[kTE], st[kTE], dg[kTE];
+#pragma unroll
+                    for (int t = 0; t < kTE; ++t) qv[t] = i0 + t < n ? load_agent(&qin[i0 + t]) : -1;
+#pragma unroll
+                    for (int t = 0; t < kTE; ++t) {
+                        st[t] = 0;
+                        dg[t] = 0;
+                        if (qv[t] >= 0) {
+                            st[t] = beg[qv[t]];
+                            dg[t] = beg[qv[t] + 1];
+                        }
+                    }
+                    int sum = 0;
+#pragma unroll
+                    for (int t = 0; t < kTE; ++t) {
+                        dg[t] -= st[t];
+                        sum += dg[t];
                     }
                     int excl, total;
-                    block_excl_scan<kW>(deg, excl, total, sm.tmp);
-                    sm.off[c.tid] = excl;
-                    sm.start[c.tid] = start;
+                    block_excl_scan<kW>(sum, excl, total, sm.tmp);
+#pragma unroll
+                    for (int t = 0; t < kTE; ++t) {
+                        sm.off[c.tid * kTE + t] = excl;
+                        sm.start[c.tid * kTE + t] = st[t];
+                        excl += dg[t];
+                    }
                     if (c.tid == 0) c.edges_td += total;
                     __syncthreads();
-                    for (int eb = 0; eb < total; eb += kGBlock) {
-                        const int e = eb + c.tid;
-                        bool want = false;
-                        int32_t w = 0;
-                        if (e < total) {
-                            int lo = 0, hi = kGBlock - 1;  // last j with off[j] <= e
-                            while (lo < hi) {
-                                const int mid = (lo + hi + 1) >> 1;
-                                if (sm.off[mid] <= e) lo = mid;
-                                else hi = mid - 1;
-                            }
-                            w = adj[sm.start[lo] + (e - sm.off[lo])];
-                            want = w != root;
+                    for (int eb = 0; eb < total; eb += kGBlock * kTK) {
+                        // lane owns edges eb + k * kGBlock + tid; the source of
+                        // edge e is the last j with off[j] <= e (off[0] = 0,
+                        // non-decreasing), found by a fixed-depth search
+                        int e[kTK], pos[kTK];
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) {
+                            e[k] = eb + k * kGBlock + c.tid;
+                            pos[k] = 0;
                         }
-                        const bool claim = wave_try_claim(want, w, allow, vis);
-                        my_found += claim;
-                        if (BW && claim) gmin = min(gmin, w);
-                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out, fedges);
+#pragma unroll
+                        for (int sp = kTile / 2; sp > 0; sp >>= 1) {
+#pragma unroll
+                            for (int k = 0; k < kTK; ++k)
+                                if (sm.off[pos[k] + sp] <= e[k]) pos[k] += sp;
+                        }
+                        bool want[kTK];
+                        int32_t w[kTK];
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) {
+                            w[k] = 0;
+                            if (e[k] < total) w[k] = adj[sm.start[pos[k]] + (e[k] - sm.off[pos[k]])];
+                        }
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) want[k] = e[k] < total && w[k] != root;
+                        bool claim[kTK];
+                        int32_t s0[kTK], d[kTK];
+                        claim_with_degree<kTK>(want, w, allow, vis, beg, claim, s0, d);
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) {
+                            my_found += claim[k];
+                            if (BW && claim[k]) gmin = min(gmin, w[k]);
+                        }
+                        push_found_n<kTK>(claim, w, s0, d, wq, qout, qn_out, hs, hout, hn_out, fedges);
                     }
                     __syncthreads();
                 } else {
@@ -408,23 +544,30 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         v = root;
                         s0 = beg[root] + (int32_t)(u - ntiles) * kGiantChunk;
                     } else {
-                        v = hin[u - ntiles].v;
-                        s0 = hin[u - ntiles].start;
+                        v = load_agent(&hin[u - ntiles].v);
+                        s0 = load_agent(&hin[u - ntiles].start);
                     }
                     const int32_t e1 = min(s0 + kGiantChunk, beg[v + 1]);
                     if (c.tid == 0) c.edges_td += e1 - s0;
-                    for (int32_t eb = s0; eb < e1; eb += kGBlock) {
-                        const int32_t e = eb + c.tid;
-                        bool want = false;
-                        int32_t w = 0;
-                        if (e < e1) {
-                            w = adj[e];
-                            want = w != root;
+                    for (int32_t eb = s0; eb < e1; eb += kGBlock * kTK) {
+                        bool want[kTK];
+                        int32_t w[kTK];
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) {
+                            const int32_t e = eb + k * kGBlock + c.tid;
+                            w[k] = e < e1 ? adj[e] : 0;
                         }
-                        const bool claim = wave_try_claim(want, w, allow, vis);
-                        my_found += claim;
-                        if (BW && claim) gmin = min(gmin, w);
-                        push_next(claim, w, beg, wq, qout, qn_out, hs, hout, hn_out, fedges);
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) want[k] = eb + k * kGBlock + c.tid < e1 && w[k] != root;
+                        bool claim[kTK];
+                        int32_t ws[kTK], wd[kTK];
+                        claim_with_degree<kTK>(want, w, allow, vis, beg, claim, ws, wd);
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k) {
+                            my_found += claim[k];
+                            if (BW && claim[k]) gmin = min(gmin, w[k]);
+                        }
+                        push_found_n<kTK>(claim, w, ws, wd, wq, qout, qn_out, hs, hout, hn_out, fedges);
                     }
                 }
             }
@@ -532,8 +675,14 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             key = (int64_t)INT32_MAX - bm;
         }
         const int64_t blk_fe = block_sum(fedges, sm);
+        // Top-down levels read what other blocks wrote in the level before
+        // through coherent loads (queue entries, hub items, counts), a
+        // candidate bitmap that is fixed during the traversal, and visited
+        // bits only as a hint for the claim atomics: their barriers skip the
+        // L2 invalidation (the acquire below catches up where needed).
         Red r;
-        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{blk, c.edges_td + c.edges_bu - edges0, blk_fe, key}, &r)) return -1;
+        if (!gsync<true, false>(gs, c.G, sm, a.dbg, Red{blk, c.edges_td + c.edges_bu - edges0, blk_fe, key}, &r))
+            return -1;
         const int64_t found = r.s0;
         if (c.b == 0 && c.tid == 0 && L < kLevelTrace) {
             gs->lv_ts[dir][L] = wall_clock64();
@@ -545,7 +694,10 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         const int64_t next_edges = r.s2;  // top-down levels: exact work of the next level
         visited += found;
         ++L;
-        if (found == 0) break;
+        if (found == 0) {
+            block_acquire();  // what follows reads the visited bits exactly
+            break;
+        }
         if (L > (int64_t)a.V + 2) {
             if (c.tid == 0) atomicOr(&gs->err.v, 2);
             return -1;
@@ -554,7 +706,10 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             const double ef = a.switch_rule ? (double)next_edges : (double)found * a.avg_degree;
             const double er = (double)(set_size - visited) * a.avg_degree;
             if (er / a.alpha < ef) mode = 1;
-        } else if (mode == 1 && ((set_size * 1.0 / a.beta) > found || L - 1 > a.gamma)) {
+        }
+        // bottom-up levels and the gather read visited bits exactly
+        if (!topdown || mode == 1) block_acquire();
+        if (!topdown && ((set_size * 1.0 / a.beta) > found || L - 1 > a.gamma)) {  // this level was bottom-up
             // gather this level's finds (vis & ~snap) as the next top-down
             // input: each wave loads 64 words at once, then expands the
             // non-empty ones
@@ -1004,9 +1159,9 @@ __global__ void k_zero_state(int64_t* gs, int64_t ngs, int64_t* ctl, int64_t nct
 }
 
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before) {
-    static_assert(sizeof(GiantState) % 8 == 0 && sizeof(Ctl) % 8 == 0, "state sizes");
+    static_assert(kGiantStateHead % 8 == 0 && sizeof(Ctl) % 8 == 0, "state sizes");
     hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(1024), 0, s, reinterpret_cast<int64_t*>(a.gs),
-                       (int64_t)(sizeof(GiantState) / 8), reinterpret_cast<int64_t*>(ctl), (int64_t)(sizeof(Ctl) / 8));
+                       (int64_t)(kGiantStateHead / 8), reinterpret_cast<int64_t*>(ctl), (int64_t)(sizeof(Ctl) / 8));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (before && (e = hipEventRecord(before, s)) != hipSuccess) return e;

@@ -481,7 +481,8 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         } else {
             CK(le);
             if (ga1) CK(hipEventRecord(ga1, s));
-            CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, sizeof(GiantState), hipMemcpyDeviceToHost, s));
+            CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, ctx->level_trace ? sizeof(GiantState) : kGiantStateHead,
+                              hipMemcpyDeviceToHost, s));
             // the kernel normally labels every vertex: count the unlabelled
             // ones now, so that one host wait covers the solve
             CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
