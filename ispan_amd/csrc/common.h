@@ -94,6 +94,12 @@ struct GiantState {
     // Last on purpose: a solve clears and copies back only what precedes
     // them, unless the level trace is on.
     int64_t lv_ts[2][kLevelTrace], lv_found[2][kLevelTrace], lv_edges[2][kLevelTrace], lv_mode[2][kLevelTrace];
+    // with GiantArgs::trace: per level, the longest and (as ~t) the shortest
+    // block work time (barrier exit to next arrival); bottom-up levels: [0]
+    // the most adjacency entries one block scanned, and the latest time any
+    // block's first wave had [1] probed its candidates, [2] scanned its
+    // misses (clock ticks from its barrier exit)
+    int64_t lv_wmax[2][kLevelTrace], lv_wmin[2][kLevelTrace], lv_cp[2][kLevelTrace][3];
 };
 constexpr size_t kGiantStateHead = offsetof(GiantState, lv_ts);
 
@@ -139,6 +145,7 @@ struct GiantArgs {
     const int32_t* roots;
     int32_t n_roots;          // 0: single graph, pivot chosen in the kernel
     const uint64_t* sel;      // optional: only these vertices take part (the rest is never active)
+    int32_t trace;            // nonzero: per-level block timings (GiantState::lv_w*, lv_smax)
     int32_t switch_rule;      // top-down -> bottom-up estimate of the frontier's work: 0 found x average
                               // degree (the reference's), 1 the frontier's exact out-degree sum
 };

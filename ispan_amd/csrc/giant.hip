@@ -29,7 +29,7 @@ constexpr int kW = kGBlock / kWave;
 enum Slot { kPivP, kPivI, kSurv, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
 static_assert(kFoundBU < kGiantSlots, "slot count");
 
-constexpr int kTileMax = kGBlock * 4;  // queue entries per top-down tile, at most
+constexpr int kTileMax = kGBlock * 1;  // queue entries per top-down tile, at most
 struct Smem {
     int32_t off[kTileMax];
     int32_t start[kTileMax];
@@ -449,6 +449,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         }
         const bool topdown = mode != 1;
         const int64_t edges0 = c.edges_td + c.edges_bu;  // thread 0's are the block's
+        const bool tr = a.trace && L < kLevelTrace;
+        const int64_t tl0 = tr ? (int64_t)wall_clock64() : 0;
         fedges = 0;
         int64_t my_found = 0;  // per thread
         if (topdown) {
@@ -579,7 +581,9 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             // words in this level.  Each stage issues its kU loads (first
             // in-neighbours, their visited bits) before any result is used.  Found vertices also go to the next queue, so
             // a switch back to top-down needs no extra pass.
-            constexpr int kU = 4;
+            constexpr int kU = 8;       // dense words per group
+            constexpr int kSparse = 8;  // at most this many candidates: a lane's own work
+            constexpr int kS = 4;       // sparse candidates per lane per step
             int scanned = 0;
             // candidates whose first probe misses are parked in the wave's LDS
             // buffer and scanned 64 at a time, one per lane, so their load
@@ -587,6 +591,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             int nmiss = 0;  // wave-uniform
             auto scan_misses = [&]() {
                 wave_sync();
+                int32_t pv = -1;  // found in the previous batch: set behind this batch's loads
                 for (int i0 = 0; i0 < nmiss; i0 += kWave) {
                     const int i = i0 + c.lane;
                     int32_t v = 0, p = 0, e = 0;
@@ -595,27 +600,52 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         p = rbeg[v] + 1;
                         e = rbeg[v + 1];
                     }
+                    if (pv >= 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + (pv >> 6)), 1ull << (pv & 63));
+                    pv = -1;
                     const bool found = wave_any_edge(i < nmiss && p < e, p, e, v, radj, is_vis, scanned);
                     if (found) {
-                        atomicOr(reinterpret_cast<unsigned long long*>(vis + (v >> 6)), 1ull << (v & 63));
+                        pv = v;
                         my_found += 1;
                         if (BW) gmin = min(gmin, v);
                     }
                 }
+                if (pv >= 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + (pv >> 6)), 1ull << (pv & 63));
                 nmiss = 0;
                 wave_sync();
             };
-            for (int64_t wb = gwave * kWave; wb < a.nwords; wb += total_waves * kWave) {
-                // lane j loads word wb + j; then only words with candidates
-                // are worked on, kU of them at a time
-                const int64_t wl = wb + c.lane;
+            // misses with more list to scan go to the wave's buffer (all lanes call)
+            auto park_misses = [&](bool miss, int32_t v) {
+                const uint64_t mm = __ballot(miss);
+                if (!mm) return;
+                if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
+                if (miss) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
+                nmiss += __popcll(mm);
+            };
+            // A wave step covers kWave / kBUrun runs of kBUrun consecutive words
+            // spread over the whole id range (run r of the step is run
+            // cb + r * total_waves + gwave), so that the id ranges where the
+            // work concentrates are shared by all blocks.
+            constexpr int kBUrun = 8;
+            const int64_t nruns = (a.nwords + kBUrun - 1) / kBUrun;
+            for (int64_t cb = 0; cb < nruns; cb += total_waves * (kWave / kBUrun)) {
+                // lane j loads one word; then only words with candidates are
+                // worked on, kU of them at a time
+                const int64_t run = cb + (int64_t)(c.lane / kBUrun) * total_waves + gwave;
+                const int64_t wl = run * kBUrun + (c.lane % kBUrun);
                 uint64_t lvw = 0, lcand = 0;
                 if (wl < a.nwords) {
                     lvw = vis[wl];
                     lcand = allow[wl] & ~lvw;
-                    st_agent(&a.snap[wl], lvw);
                 }
-                uint64_t busy = __ballot(lcand != 0);
+                // words with more than kSparse candidates: the wave works on
+                // kU of them at a time, one lane per vertex
+                const bool dense = __popcll(lcand) > kSparse;
+                uint64_t busy = __ballot(dense);
+                // Found bits of group member u wait in lane u and go out as
+                // one atomic instruction behind the next group's loads: a
+                // load's wait also waits for the atomics issued before it.
+                int64_t pw = 0;
+                uint64_t pm = 0;
                 while (busy) {
                     int js[kU];
 #pragma unroll
@@ -624,12 +654,16 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         busy &= busy ? busy - 1 : 0;
                     }
                     int32_t f[kU];
+                    int64_t ws[kU];  // word of group member u (wave-uniform)
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         f[u] = -1;
+                        ws[u] = js[u] >= 0 ? __shfl(wl, js[u], kWave) : 0;
                         if (js[u] >= 0 && ((__shfl(lcand, js[u], kWave) >> c.lane) & 1ull))
-                            f[u] = rfirst[(wb + js[u]) * 64 + c.lane];
+                            f[u] = rfirst[ws[u] * 64 + c.lane];
                     }
+                    if (pm) atomicOr(reinterpret_cast<unsigned long long*>(vis + pw), (unsigned long long)pm);
+                    pm = 0;
                     uint32_t hits = 0;
 #pragma unroll
                     for (int u = 0; u < kU; ++u) hits |= (f[u] >= 0 && bit_of(vis, f[u])) ? 1u << u : 0u;
@@ -637,27 +671,67 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     for (int u = 0; u < kU; ++u) {
                         if (js[u] < 0) continue;
                         const uint64_t cm = __shfl(lcand, js[u], kWave);
-                        const int64_t w = wb + js[u];
+                        const int64_t w = ws[u];
                         const int32_t v = (int32_t)(w * 64 + c.lane);
                         const bool is_c = (cm >> c.lane) & 1ull;
                         const bool hit = is_c && ((hits >> u) & 1u);
                         scanned += is_c;
-                        const uint64_t mm = __ballot(is_c && !hit && f[u] >= 0);  // misses with more list to scan
-                        if (mm) {
-                            if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
-                            if ((mm >> c.lane) & 1ull) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
-                            nmiss += __popcll(mm);
-                        }
+                        park_misses(is_c && !hit && f[u] >= 0, v);
+                        // found bits are set within the level: later probes
+                        // already see them
                         const uint64_t fm = __ballot(hit);
-                        if (fm && c.lane == 0)
-                            atomicOr(reinterpret_cast<unsigned long long*>(vis + w), (unsigned long long)fm);
+                        if (c.lane == u) {
+                            pw = w;
+                            pm = fm;
+                        }
                         my_found += hit;
                         if (BW && hit) gmin = min(gmin, v);
                     }
                 }
+                // sparse words: each lane works through its own word's
+                // candidates, kS at a time, and sets its found bits at once
+                uint64_t mine = dense ? 0ull : lcand, fm_lane = 0;
+                while (__ballot(mine != 0)) {
+                    int32_t v[kS], f[kS];
+#pragma unroll
+                    for (int k = 0; k < kS; ++k) {
+                        v[k] = -1;
+                        if (mine) {
+                            v[k] = (int32_t)(wl * 64 + __ffsll((unsigned long long)mine) - 1);
+                            mine &= mine - 1;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kS; ++k) f[k] = v[k] >= 0 ? rfirst[v[k]] : -1;
+                    bool hit[kS];
+#pragma unroll
+                    for (int k = 0; k < kS; ++k) hit[k] = f[k] >= 0 && bit_of(vis, f[k]);
+#pragma unroll
+                    for (int k = 0; k < kS; ++k) {
+                        scanned += v[k] >= 0;
+                        park_misses(v[k] >= 0 && !hit[k] && f[k] >= 0, v[k]);
+                        if (hit[k]) {
+                            fm_lane |= 1ull << (v[k] & 63);
+                            my_found += 1;
+                            if (BW) gmin = min(gmin, v[k]);
+                        }
+                    }
+                }
+                if (pm) atomicOr(reinterpret_cast<unsigned long long*>(vis + pw), (unsigned long long)pm);
+                if (fm_lane) atomicOr(reinterpret_cast<unsigned long long*>(vis + wl), (unsigned long long)fm_lane);
+                if (wl < a.nwords) st_agent(&a.snap[wl], lvw);  // the gather's "visited before this level"
             }
+            if (tr && c.tid == 0)  // wave 0 of each block
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][1]),
+                          (unsigned long long)((int64_t)wall_clock64() - tl0));
             scan_misses();
-            c.edges_bu += block_sum(scanned, sm);
+            if (tr && c.tid == 0)
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][2]),
+                          (unsigned long long)((int64_t)wall_clock64() - tl0));
+            const int64_t bscan = block_sum(scanned, sm);
+            c.edges_bu += bscan;
+            if (tr && c.tid == 0)
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][0]), (unsigned long long)bscan);
         }
         // ---- level totals: every block computes the same sums
         const int64_t blk = block_sum(my_found, sm);
@@ -675,6 +749,11 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             key = (int64_t)INT32_MAX - bm;
         }
         const int64_t blk_fe = block_sum(fedges, sm);
+        if (tr && c.tid == 0) {
+            const unsigned long long wt = (unsigned long long)((int64_t)wall_clock64() - tl0);
+            atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmax[dir][L]), wt);
+            atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmin[dir][L]), ~wt);
+        }
         // Top-down levels read what other blocks wrote in the level before
         // through coherent loads (queue entries, hub items, counts), a
         // candidate bitmap that is fixed during the traversal, and visited
@@ -917,6 +996,10 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     // loads before it uses any of them; lane i loads beg[v] once and takes
     // beg[v + 1] from lane i + 1 (lane 63 from the next word's lane 0)
     constexpr int kUA = 8;
+    // lane u < kUA holds the previous step's word w0 + u until the next
+    // step's loads are out: a load's wait also waits for earlier stores
+    int64_t pword = -1;
+    uint64_t pmine = 0;
     for (int64_t w0 = gwave * kUA; w0 < a.nwords; w0 += total_waves * kUA) {
         int32_t fo[kUA], bo[kUA];
 #pragma unroll
@@ -931,6 +1014,15 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
             fl = a.fwb[vl];
             bl = a.bwb[vl];
         }
+        // vertices outside the selection never become active: lane u < kUA
+        // loads word w0 + u's selection
+        const uint64_t lsel = (c.lane < kUA && w0 + c.lane < a.nwords) ? (a.sel ? a.sel[w0 + c.lane] : ~0ull) : 0ull;
+        if (pword >= 0) {
+            st_agent(&a.act[pword], pmine);
+            st_agent(&a.fvis[pword], (uint64_t)0);
+            st_agent(&a.bvis[pword], (uint64_t)0);
+            pword = -1;
+        }
         uint64_t mine = 0;  // lane u < kUA keeps word w0 + u's survivors
 #pragma unroll
         for (int u = 0; u < kUA; ++u) {
@@ -942,8 +1034,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
                 fn = f0;
                 bn = b0;
             }
-            // vertices outside the selection never become active
-            const uint64_t sw = w < a.nwords ? (a.sel ? a.sel[w] : ~0ull) : 0ull;
+            const uint64_t sw = __shfl(lsel, u, kWave);
             const bool on = v < a.V && ((sw >> c.lane) & 1ull);
             const int32_t od = fn - fo[u], id = bn - bo[u];
             const bool s = on && od > 0 && id > 0;
@@ -958,11 +1049,15 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
             if (c.lane == u) mine = m;
         }
         if (c.lane < kUA && w0 + c.lane < a.nwords) {
-            st_agent(&a.act[w0 + c.lane], mine);
-            st_agent(&a.fvis[w0 + c.lane], (uint64_t)0);
-            st_agent(&a.bvis[w0 + c.lane], (uint64_t)0);
+            pword = w0 + c.lane;
+            pmine = mine;
             surv += __popcll(mine);
         }
+    }
+    if (pword >= 0) {
+        st_agent(&a.act[pword], pmine);
+        st_agent(&a.fvis[pword], (uint64_t)0);
+        st_agent(&a.bvis[pword], (uint64_t)0);
     }
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) {
@@ -1161,7 +1256,8 @@ __global__ void k_zero_state(int64_t* gs, int64_t ngs, int64_t* ctl, int64_t nct
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before) {
     static_assert(kGiantStateHead % 8 == 0 && sizeof(Ctl) % 8 == 0, "state sizes");
     hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(1024), 0, s, reinterpret_cast<int64_t*>(a.gs),
-                       (int64_t)(kGiantStateHead / 8), reinterpret_cast<int64_t*>(ctl), (int64_t)(sizeof(Ctl) / 8));
+                       (int64_t)((a.trace ? sizeof(GiantState) : kGiantStateHead) / 8), reinterpret_cast<int64_t*>(ctl),
+                       (int64_t)(sizeof(Ctl) / 8));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (before && (e = hipEventRecord(before, s)) != hipSuccess) return e;

@@ -469,6 +469,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.parent = ctx->parent;
         ga.fq_n_ctl = ctx->ctl->fq_n;
         ga.dbg = ctx->gdbg;
+        ga.trace = ctx->level_trace ? 1 : 0;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
         if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) ctx->recs.push_back({3, ga0, ga1});
         const hipError_t le = launch_giant(ga, ctx->ctl, ctx->coop_grid, s, ga0);
@@ -539,9 +540,13 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                     int64_t t = d == 0 ? gs.ts[1] : gs.ts[2];
                     const int64_t nl = std::min<int64_t>(d == 0 ? gs.fw_levels : gs.bw_levels, kLevelTrace);
                     for (int64_t l = 0; l < nl; ++l) {
-                        fprintf(stderr, "[ispan] %s level %2lld %s found %10lld edges %11lld  %8.3f ms\n", d ? "BW" : "FW",
-                                (long long)l, gs.lv_mode[d][l] ? "BU" : "TD", (long long)gs.lv_found[d][l],
-                                (long long)gs.lv_edges[d][l], (gs.lv_ts[d][l] - t) * k);
+                        fprintf(stderr,
+                                "[ispan] %s level %2lld %s found %10lld edges %11lld  %8.3f ms  work %.3f..%.3f "
+                                "max block scan %.0f marks %.3f %.3f\n",
+                                d ? "BW" : "FW", (long long)l, gs.lv_mode[d][l] ? "BU" : "TD",
+                                (long long)gs.lv_found[d][l], (long long)gs.lv_edges[d][l], (gs.lv_ts[d][l] - t) * k,
+                                (int64_t)~(uint64_t)gs.lv_wmin[d][l] * k, gs.lv_wmax[d][l] * k, (double)gs.lv_cp[d][l][0],
+                                gs.lv_cp[d][l][1] * k, gs.lv_cp[d][l][2] * k);
                         t = gs.lv_ts[d][l];
                     }
                 }
