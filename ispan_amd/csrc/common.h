@@ -98,8 +98,14 @@ struct GiantState {
     // block work time (barrier exit to next arrival); bottom-up levels: [0]
     // the most adjacency entries one block scanned, and the latest time any
     // block's first wave had [1] probed its candidates, [2] scanned its
-    // misses (clock ticks from its barrier exit)
-    int64_t lv_wmax[2][kLevelTrace], lv_wmin[2][kLevelTrace], lv_cp[2][kLevelTrace][3];
+    // misses, [3] finished its dense words (clock ticks from its barrier exit)
+    int64_t lv_wmax[2][kLevelTrace], lv_wmin[2][kLevelTrace], lv_cp[2][kLevelTrace][4];
+    // with GiantArgs::trace: each block's work time in the first bottom-up
+    // level of the forward traversal, and the adjacency entries it scanned
+    int64_t bu1_work[1024], bu1_scan[1024];
+    // with GiantArgs::trace: latest clock of any block after the post
+    // phase's label pass, its barrier, and the active queue rebuild
+    int64_t post_cp[3];
 };
 constexpr size_t kGiantStateHead = offsetof(GiantState, lv_ts);
 
@@ -145,7 +151,8 @@ struct GiantArgs {
     const int32_t* roots;
     int32_t n_roots;          // 0: single graph, pivot chosen in the kernel
     const uint64_t* sel;      // optional: only these vertices take part (the rest is never active)
-    int32_t trace;            // nonzero: per-level block timings (GiantState::lv_w*, lv_smax)
+    int32_t trace;            // nonzero: per-level block timings (GiantState::lv_w*, lv_cp, bu1_*)
+    int64_t bu_mult;          // coprime to grid * waves per block: bottom-up run permutation
     int32_t switch_rule;      // top-down -> bottom-up estimate of the frontier's work: 0 found x average
                               // degree (the reference's), 1 the frontier's exact out-degree sum
 };
@@ -156,6 +163,8 @@ constexpr int kGiantChunk = 2048;   // edges per work item
 inline int64_t giant_hub_capacity(int64_t E) { return 5 * (E / kGiantChunk) + 64; }
 
 int giant_max_blocks_per_cu();
+// GiantArgs::bu_mult for a grid of `grid` blocks
+int64_t giant_bu_mult(int grid);
 // zeroes a.gs and ctl, records `before` (when given), then launches k_giant
 // cooperatively
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before);

@@ -35,6 +35,16 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// v as held by lane l, where l is the same in every lane: one v_readlane
+// per 32 bits (__shfl would route through the LDS crossbar).
+__device__ __forceinline__ int32_t lane_get(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint64_t lane_get(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t lane_get(int64_t v, int l) { return (int64_t)lane_get((uint64_t)v, l); }
+
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = lane_id();
 #pragma unroll
@@ -68,8 +78,8 @@ __device__ __forceinline__ void block_excl_scan(int x, int& excl, int& total, in
     const int t = lane_id() < NW ? s_tmp[lane_id()] : 0;
     const int ti = wave_incl_scan(t);
     const int wv = wave_id();
-    excl = __shfl(ti - t, wv, kWave) + incl - x;
-    total = __shfl(ti, NW - 1, kWave);
+    excl = lane_get(ti - t, wv) + incl - x;
+    total = lane_get(ti, NW - 1);
     __syncthreads();
 }
 
@@ -88,7 +98,7 @@ __device__ __forceinline__ void wq_flush(WaveQueue& wq, int32_t* gq, int32_t* gc
     wave_sync();
     int base = 0;
     if (lane_id() == 0) base = atomicAdd(gcount, wq.n);
-    base = __shfl(base, 0, kWave);
+    base = lane_get(base, 0);
     for (int i = lane_id(); i < wq.n; i += kWave) {
         if (COH) st_agent(&gq[base + i], wq.buf[i]);
         else gq[base + i] = wq.buf[i];
@@ -180,7 +190,7 @@ __device__ __forceinline__ bool wave_any_edge(bool active, int32_t p, int32_t en
     uint64_t pend = __ballot(active && !found && p < end);
     while (pend) {
         const int l = __ffsll((unsigned long long)pend) - 1;
-        const int32_t pp = __shfl(p, l, kWave), ee = __shfl(end, l, kWave), ss = __shfl(self, l, kWave);
+        const int32_t pp = lane_get(p, l), ee = lane_get(end, l), ss = lane_get(self, l);
         bool hit = false;
         int32_t qb = pp;
         for (; qb < ee; qb += kSteps * kWave) {

@@ -169,7 +169,7 @@ __device__ __forceinline__ int64_t block_sum(int64_t x, Smem& sm) {
     x = lane_id() < kW ? sm.red[lane_id()] : 0;
 #pragma unroll
     for (int d = kW / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    x = __shfl(x, 0, kWave);
+    x = lane_get(x, 0);
     __syncthreads();
     return x;
 }
@@ -204,7 +204,7 @@ __device__ __forceinline__ void hub_flush(HubStage& hs, HubItem* hout, int32_t* 
     const int incl = wave_incl_scan(nch);
     int base = 0;
     if (lane == kWave - 1) base = atomicAdd(hn_out, incl);
-    base = __shfl(base, kWave - 1, kWave) + incl - nch;
+    base = lane_get(base, kWave - 1) + incl - nch;
     for (int k = 0; k < nch; ++k) {
         st_agent(&hout[base + k].v, v);
         st_agent(&hout[base + k].start, s0 + k * kGiantChunk);
@@ -265,7 +265,7 @@ __device__ __forceinline__ void block_hub_flush(Ctx& c, HubStage& hs, HubItem* h
         nch = (hs.d[c.lane] + kGiantChunk - 1) / kGiantChunk;
     }
     const int incl = wave_incl_scan(nch);
-    const int base = block_reserve(c, __shfl(incl, kWave - 1, kWave), hn_out) + incl - nch;
+    const int base = block_reserve(c, lane_get(incl, kWave - 1), hn_out) + incl - nch;
     for (int k = 0; k < nch; ++k) {
         st_agent(&hout[base + k].v, v);
         st_agent(&hout[base + k].start, s0 + k * kGiantChunk);
@@ -457,10 +457,15 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             // ---- top-down: tiles of the queue, then hub work items; level 0
             // expands the root's list in kGiantChunk-edge pieces.
             int64_t n = 0, nh = 0;
+            int32_t rchunk = kGiantChunk;  // level 0: edges per piece of the root's list
             if (L == 0 && !multi) {
                 if (c.b == 0 && c.tid == 0)
                     atomicOr(reinterpret_cast<unsigned long long*>(vis + (root >> 6)), 1ull << (root & 63));
-                nh = (beg[root + 1] - beg[root] + kGiantChunk - 1) / kGiantChunk;
+                // the root's list in about one piece per block
+                const int32_t rdeg = beg[root + 1] - beg[root];
+                rchunk = max(256, ((rdeg + c.G - 1) / c.G + 63) & ~63);
+                rchunk = min(rchunk, kGiantChunk);
+                nh = (rdeg + rchunk - 1) / rchunk;
             } else {
                 n = load_agent(&gs->qn[dir][L % 3].v);
                 nh = load_agent(&gs->hn[dir][L % 3].v);
@@ -544,12 +549,12 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     int32_t v, s0;
                     if (L == 0 && !multi) {
                         v = root;
-                        s0 = beg[root] + (int32_t)(u - ntiles) * kGiantChunk;
+                        s0 = beg[root] + (int32_t)(u - ntiles) * rchunk;
                     } else {
                         v = load_agent(&hin[u - ntiles].v);
                         s0 = load_agent(&hin[u - ntiles].start);
                     }
-                    const int32_t e1 = min(s0 + kGiantChunk, beg[v + 1]);
+                    const int32_t e1 = min(s0 + (L == 0 && !multi ? rchunk : kGiantChunk), beg[v + 1]);
                     if (c.tid == 0) c.edges_td += e1 - s0;
                     for (int32_t eb = s0; eb < e1; eb += kGBlock * kTK) {
                         bool want[kTK];
@@ -627,10 +632,18 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             // work concentrates are shared by all blocks.
             constexpr int kBUrun = 8;
             const int64_t nruns = (a.nwords + kBUrun - 1) / kBUrun;
+            // All waves sweep the id range in the same order (band by band:
+            // vertices found early help later probes of the level hit), and
+            // within a band wave g takes run (g * bu_mult) mod total_waves, a
+            // permutation: with plain indices a block's runs would share
+            // fixed bits of their vertex ids, and in graphs whose degrees
+            // follow id bits (R-MAT) the blocks' work would differ severalfold.
+            // (bu_mult < total_waves <= 65536: the product fits 32 bits)
+            const int64_t pg = (int64_t)(((uint32_t)gwave * (uint32_t)a.bu_mult) % (uint32_t)total_waves);
             for (int64_t cb = 0; cb < nruns; cb += total_waves * (kWave / kBUrun)) {
                 // lane j loads one word; then only words with candidates are
                 // worked on, kU of them at a time
-                const int64_t run = cb + (int64_t)(c.lane / kBUrun) * total_waves + gwave;
+                const int64_t run = cb + (int64_t)(c.lane / kBUrun) * total_waves + pg;
                 const int64_t wl = run * kBUrun + (c.lane % kBUrun);
                 uint64_t lvw = 0, lcand = 0;
                 if (wl < a.nwords) {
@@ -658,8 +671,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         f[u] = -1;
-                        ws[u] = js[u] >= 0 ? __shfl(wl, js[u], kWave) : 0;
-                        if (js[u] >= 0 && ((__shfl(lcand, js[u], kWave) >> c.lane) & 1ull))
+                        ws[u] = js[u] >= 0 ? lane_get(wl, js[u]) : 0;
+                        if (js[u] >= 0 && ((lane_get(lcand, js[u]) >> c.lane) & 1ull))
                             f[u] = rfirst[ws[u] * 64 + c.lane];
                     }
                     if (pm) atomicOr(reinterpret_cast<unsigned long long*>(vis + pw), (unsigned long long)pm);
@@ -670,7 +683,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         if (js[u] < 0) continue;
-                        const uint64_t cm = __shfl(lcand, js[u], kWave);
+                        const uint64_t cm = lane_get(lcand, js[u]);
                         const int64_t w = ws[u];
                         const int32_t v = (int32_t)(w * 64 + c.lane);
                         const bool is_c = (cm >> c.lane) & 1ull;
@@ -688,6 +701,9 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                         if (BW && hit) gmin = min(gmin, v);
                     }
                 }
+                if (tr && c.tid == 0 && cb == 0)
+                    atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][3]),
+                              (unsigned long long)((int64_t)wall_clock64() - tl0));
                 // sparse words: each lane works through its own word's
                 // candidates, kS at a time, and sets its found bits at once
                 uint64_t mine = dense ? 0ull : lcand, fm_lane = 0;
@@ -730,8 +746,13 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                           (unsigned long long)((int64_t)wall_clock64() - tl0));
             const int64_t bscan = block_sum(scanned, sm);
             c.edges_bu += bscan;
-            if (tr && c.tid == 0)
+            if (tr && c.tid == 0) {
                 atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][0]), (unsigned long long)bscan);
+                if (!BW && c.b < 1024 && gs->bu1_scan[c.b] == 0) {
+                    gs->bu1_scan[c.b] = bscan + 1;
+                    gs->bu1_work[c.b] = (int64_t)wall_clock64() - tl0;
+                }
+            }
         }
         // ---- level totals: every block computes the same sums
         const int64_t blk = block_sum(my_found, sm);
@@ -745,7 +766,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             int32_t bm = c.lane < kW ? sm.tmp[c.lane] : INT32_MAX;
 #pragma unroll
             for (int d = kW / 2; d > 0; d >>= 1) bm = min(bm, __shfl_xor(bm, d, kWave));
-            bm = __shfl(bm, 0, kWave);
+            bm = lane_get(bm, 0);
             key = (int64_t)INT32_MAX - bm;
         }
         const int64_t blk_fe = block_sum(fedges, sm);
@@ -804,7 +825,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                 while (busy) {
                     const int l = __ffsll((unsigned long long)busy) - 1;
                     busy &= busy - 1;
-                    const uint64_t m = __shfl(nw, l, kWave);
+                    const uint64_t m = lane_get(nw, l);
                     const int32_t v = (int32_t)((wb + l) * 64 + c.lane);
                     push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next, fedges);
                 }
@@ -1027,14 +1048,14 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
 #pragma unroll
         for (int u = 0; u < kUA; ++u) {
             const int64_t w = w0 + u, v = w * 64 + c.lane;
-            const int32_t f0 = u + 1 < kUA ? __shfl(fo[u + 1 < kUA ? u + 1 : u], 0, kWave) : fl;
-            const int32_t b0 = u + 1 < kUA ? __shfl(bo[u + 1 < kUA ? u + 1 : u], 0, kWave) : bl;
+            const int32_t f0 = u + 1 < kUA ? lane_get(fo[u + 1 < kUA ? u + 1 : u], 0) : fl;
+            const int32_t b0 = u + 1 < kUA ? lane_get(bo[u + 1 < kUA ? u + 1 : u], 0) : bl;
             int32_t fn = __shfl_down(fo[u], 1, kWave), bn = __shfl_down(bo[u], 1, kWave);
             if (c.lane == kWave - 1) {
                 fn = f0;
                 bn = b0;
             }
-            const uint64_t sw = __shfl(lsel, u, kWave);
+            const uint64_t sw = lane_get(lsel, u);
             const bool on = v < a.V && ((sw >> c.lane) & 1ull);
             const int32_t od = fn - fo[u], id = bn - bo[u];
             const bool s = on && od > 0 && id > 0;
@@ -1158,7 +1179,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         }
         const int nw = (int)min<int64_t>(kWave, we - base);
         for (int j = 0; j < nw; ++j) {
-            const uint64_t aj = __shfl(a0, j, kWave), gj = __shfl(g, j, kWave), sj = __shfl(sw, j, kWave);
+            const uint64_t aj = lane_get(a0, j), gj = lane_get(g, j), sj = lane_get(sw, j);
             const int64_t v = (base + j) * 64 + c.lane;
             if (v < a.V && ((sj >> c.lane) & 1ull)) {
                 const bool gi = (gj >> c.lane) & 1ull, ac = (aj >> c.lane) & 1ull;
@@ -1167,6 +1188,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         }
     }
     const int64_t cnt_blk = block_sum(cnt, sm);
+    if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[0], (unsigned long long)wall_clock64());
     if (c.tid == 0) {
         st_agent(&a.part[(int64_t)kPost * c.G + c.b], (int64_t)(cnt_blk));
         st_agent(&a.part[(int64_t)kEdgeTD * c.G + c.b], (int64_t)(c.edges_td));
@@ -1175,6 +1197,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         st_agent(&a.part[(int64_t)kFoundBU * c.G + c.b], (int64_t)(c.found_bu));
     }
     if (!gsync(gs, c.G, sm, a.dbg)) return;
+    if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[1], (unsigned long long)wall_clock64());
     // exclusive prefix of the block counts, then an ordered block-local scan
     int64_t off, tot;
     {
@@ -1201,6 +1224,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         }
         off += total;
     }
+    if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[2], (unsigned long long)wall_clock64());
     if (c.b == 0) {
         const int64_t w0 = sum_slot(a.part, kEdgeTD, c.G, sm), w1 = sum_slot(a.part, kEdgeBU, c.G, sm);
         const int64_t w2 = sum_slot(a.part, kFoundTD, c.G, sm), w3 = sum_slot(a.part, kFoundBU, c.G, sm);
@@ -1251,6 +1275,20 @@ int giant_max_blocks_per_cu() {
 __global__ void k_zero_state(int64_t* gs, int64_t ngs, int64_t* ctl, int64_t nctl) {
     for (int64_t i = threadIdx.x; i < ngs; i += blockDim.x) gs[i] = 0;
     for (int64_t i = threadIdx.x; i < nctl; i += blockDim.x) ctl[i] = 0;
+}
+
+int64_t giant_bu_mult(int grid) {
+    const int64_t t = (int64_t)grid * kW;
+    int64_t m = 2654435761ll % t;  // Knuth's multiplicative hash constant, then the next value coprime to t
+    for (;; ++m) {
+        int64_t x = m, y = t;
+        while (y) {
+            const int64_t r = x % y;
+            x = y;
+            y = r;
+        }
+        if (x == 1) return m;
+    }
 }
 
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before) {

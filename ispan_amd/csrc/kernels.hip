@@ -82,10 +82,10 @@ __global__ __launch_bounds__(kBlock) void k_trim1_first(const int32_t* __restric
         if (wave == 0) {
             const int c = s_off[lane];
             const int incl = wave_incl_scan(c);
-            const int total = __shfl(incl, kWave - 1, kWave);
+            const int total = lane_get(incl, kWave - 1);
             int b = 0;
             if (lane == 0 && total) b = atomicAdd(&ctl->fq_n[0], total);
-            b = __shfl(b, 0, kWave);
+            b = lane_get(b, 0);
             s_off[lane] = b + incl - c;
         }
         __syncthreads();

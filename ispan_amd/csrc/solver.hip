@@ -470,6 +470,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.fq_n_ctl = ctx->ctl->fq_n;
         ga.dbg = ctx->gdbg;
         ga.trace = ctx->level_trace ? 1 : 0;
+        ga.bu_mult = giant_bu_mult(ctx->coop_grid);
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
         if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) ctx->recs.push_back({3, ga0, ga1});
         const hipError_t le = launch_giant(ga, ctx->ctl, ctx->coop_grid, s, ga0);
@@ -536,17 +537,40 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                         ((int64_t)~gs.t_arrive[0] - gs.ts[0]) * k0, (gs.t_arrive[1] - gs.ts[0]) * k0,
                         (gs.ts[1] - gs.ts[0]) * k0);
                 const double k = 1.0 / ctx->clock_khz;
+                if (gs.post_cp[0])
+                    fprintf(stderr, "[ispan] post after BW: labels %.3f ms, barrier %.3f ms, queue %.3f ms\n",
+                            (gs.post_cp[0] - gs.ts[3]) * k, (gs.post_cp[1] - gs.ts[3]) * k,
+                            (gs.post_cp[2] - gs.ts[3]) * k);
+                if (gs.bu1_scan[0]) {  // first forward bottom-up level, per block and by XCC (block % 8)
+                    const int nb = std::min(ctx->coop_grid, 1024);
+                    double xs[8] = {0}, xw[8] = {0}, xmax[8] = {0};
+                    int xn[8] = {0};
+                    for (int b = 0; b < nb; ++b) {
+                        const int x = b % 8;
+                        xs[x] += (double)(gs.bu1_scan[b] - 1);
+                        xw[x] += gs.bu1_work[b] * k;
+                        xmax[x] = std::max(xmax[x], gs.bu1_work[b] * k);
+                        ++xn[x];
+                    }
+                    for (int x = 0; x < 8; ++x)
+                        if (xn[x])
+                            fprintf(stderr, "[ispan] BU1 xcc %d: mean work %.3f ms max %.3f ms, mean scan %.0f\n", x,
+                                    xw[x] / xn[x], xmax[x], xs[x] / xn[x]);
+                    fprintf(stderr, "[ispan] BU1 work by block:");
+                    for (int b = 0; b < nb; ++b) fprintf(stderr, "%s%.0f", b % 32 ? " " : "\n  ", gs.bu1_work[b] * k * 1e3);
+                    fprintf(stderr, "\n");
+                }
                 for (int d = 0; d < 2; ++d) {
                     int64_t t = d == 0 ? gs.ts[1] : gs.ts[2];
                     const int64_t nl = std::min<int64_t>(d == 0 ? gs.fw_levels : gs.bw_levels, kLevelTrace);
                     for (int64_t l = 0; l < nl; ++l) {
                         fprintf(stderr,
                                 "[ispan] %s level %2lld %s found %10lld edges %11lld  %8.3f ms  work %.3f..%.3f "
-                                "max block scan %.0f marks %.3f %.3f\n",
+                                "max block scan %.0f marks dense %.3f probed %.3f misses %.3f\n",
                                 d ? "BW" : "FW", (long long)l, gs.lv_mode[d][l] ? "BU" : "TD",
                                 (long long)gs.lv_found[d][l], (long long)gs.lv_edges[d][l], (gs.lv_ts[d][l] - t) * k,
                                 (int64_t)~(uint64_t)gs.lv_wmin[d][l] * k, gs.lv_wmax[d][l] * k, (double)gs.lv_cp[d][l][0],
-                                gs.lv_cp[d][l][1] * k, gs.lv_cp[d][l][2] * k);
+                                gs.lv_cp[d][l][3] * k, gs.lv_cp[d][l][1] * k, gs.lv_cp[d][l][2] * k);
                         t = gs.lv_ts[d][l];
                     }
                 }
