@@ -45,26 +45,54 @@ __device__ __forceinline__ uint64_t lane_get(uint64_t v, int l) {
 }
 __device__ __forceinline__ int64_t lane_get(int64_t v, int l) { return (int64_t)lane_get((uint64_t)v, l); }
 
+// x of lane (lane - N) in the same 16-lane row (DPP row_shr); lanes
+// without one get `fill`.
+template <int N>
+__device__ __forceinline__ int32_t row_shr(int32_t x, int32_t fill) {
+    return __builtin_amdgcn_update_dpp(fill, x, 0x110 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ __forceinline__ uint64_t row_shr(uint64_t x, uint64_t fill) {
+    const uint32_t lo = (uint32_t)row_shr<N>((int32_t)(uint32_t)x, (int32_t)(uint32_t)fill);
+    const uint32_t hi = (uint32_t)row_shr<N>((int32_t)(uint32_t)(x >> 32), (int32_t)(uint32_t)(fill >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+template <int N>
+__device__ __forceinline__ int64_t row_shr(int64_t x, int64_t fill) {
+    return (int64_t)row_shr<N>((uint64_t)x, (uint64_t)fill);
+}
+
+// Inclusive scan of op within each 16-lane row (lane 15 of a row holds the
+// row's total); id is op's identity.
+template <class T, class Op>
+__device__ __forceinline__ T row_incl(T x, T id, Op op) {
+    x = op(x, row_shr<1>(x, id));
+    x = op(x, row_shr<2>(x, id));
+    x = op(x, row_shr<4>(x, id));
+    x = op(x, row_shr<8>(x, id));
+    return x;
+}
+
+// Wave-wide inclusive sum: row scans, then the totals of earlier rows.
 __device__ __forceinline__ int wave_incl_scan(int x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        int y = __shfl_up(x, d, kWave);
-        if (lane >= d) x += y;
-    }
-    return x;
+    x = row_incl(x, 0, [](int a, int b) { return a + b; });
+    const int t0 = lane_get(x, 15), t1 = lane_get(x, 31), t2 = lane_get(x, 47);
+    const int r = lane_id() >> 4;
+    return x + (r >= 1 ? t0 : 0) + (r >= 2 ? t1 : 0) + (r >= 3 ? t2 : 0);
 }
 
+// Wave-wide sum and minimum; every lane gets the result.
 __device__ __forceinline__ int wave_sum(int x) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    return x;
+    x = row_incl(x, 0, [](int a, int b) { return a + b; });
+    return lane_get(x, 15) + lane_get(x, 31) + lane_get(x, 47) + lane_get(x, 63);
 }
-
+__device__ __forceinline__ int64_t wave_sum(int64_t x) {
+    x = row_incl(x, (int64_t)0, [](int64_t a, int64_t b) { return a + b; });
+    return lane_get(x, 15) + lane_get(x, 31) + lane_get(x, 47) + lane_get(x, 63);
+}
 __device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x = min(x, __shfl_xor(x, d, kWave));
-    return x;
+    x = row_incl(x, INT32_MAX, [](int a, int b) { return min(a, b); });
+    return min(min(lane_get(x, 15), lane_get(x, 31)), min(lane_get(x, 47), lane_get(x, 63)));
 }
 
 // Block-wide exclusive scan (256 threads); every thread gets the total.

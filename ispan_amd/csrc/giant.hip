@@ -162,14 +162,12 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
 
 // Sum of a per-thread value over the block (every thread gets it).
 __device__ __forceinline__ int64_t block_sum(int64_t x, Smem& sm) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    static_assert(kW == 16, "one DPP row holds the wave totals");
+    x = wave_sum(x);
     if (lane_id() == 0) sm.red[wave_id()] = x;
     __syncthreads();
-    x = lane_id() < kW ? sm.red[lane_id()] : 0;
-#pragma unroll
-    for (int d = kW / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    x = lane_get(x, 0);
+    x = row_incl(sm.red[lane_id() & 15], (int64_t)0, [](int64_t a, int64_t b) { return a + b; });
+    x = lane_get(x, 15);
     __syncthreads();
     return x;
 }
@@ -324,19 +322,37 @@ __device__ __forceinline__ void claim_with_degree(const bool (&want)[K], const i
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int32_t key = cand[k] ? (w[k] >> 6) : -1 - lane;  // lanes without a candidate: runs of their own
-        const int32_t pkey = __shfl_up(key, 1, kWave);
+        // key of lane - 1: DPP within rows, the three row starts by readlane
+        int32_t pkey = row_shr<1>(key, key);
+        {
+            const int32_t k15 = lane_get(key, 15), k31 = lane_get(key, 31), k47 = lane_get(key, 47);
+            pkey = lane == 16 ? k15 : lane == 32 ? k31 : lane == 48 ? k47 : pkey;
+        }
         const uint64_t heads = __ballot(lane == 0 || pkey != key);
         const uint64_t le = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);
         const int start = 63 - __clzll(heads & le);
         const uint64_t after = heads & ~le;
         end[k] = after ? __ffsll((unsigned long long)after) - 2 : kWave - 1;
+        // OR over lanes start..lane: scans within rows, then the carries of
+        // runs that cross a row boundary, row by row
         uint64_t in = cand[k] ? 1ull << (w[k] & 63) : 0ull;
-#pragma unroll
-        for (int dd = 1; dd < kWave; dd <<= 1) {
-            const uint64_t y = __shfl_up(in, dd, kWave);
-            if (lane - dd >= start) in |= y;
-        }
-        uint64_t ex = __shfl_up(in, 1, kWave);
+        uint64_t y = row_shr<1>(in, 0ull);
+        if (lane - 1 >= start) in |= y;
+        y = row_shr<2>(in, 0ull);
+        if (lane - 2 >= start) in |= y;
+        y = row_shr<4>(in, 0ull);
+        if (lane - 4 >= start) in |= y;
+        y = row_shr<8>(in, 0ull);
+        if (lane - 8 >= start) in |= y;
+        const uint64_t c15 = lane_get(in, 15);
+        if (lane >= 16 && lane < 32 && start <= 15) in |= c15;
+        const uint64_t c31 = lane_get(in, 31);
+        if (lane >= 32 && lane < 48 && start <= 31) in |= c31;
+        const uint64_t c47 = lane_get(in, 47);
+        if (lane >= 48 && start <= 47) in |= c47;
+        // the run's OR before this lane
+        uint64_t ex = row_shr<1>(in, 0ull);
+        ex = lane == 16 ? c15 : lane == 32 ? c31 : lane == 48 ? c47 : ex;
         if (lane == start) ex = 0;
         incl[k] = in;
         excl[k] = ex;
@@ -763,10 +779,8 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             const int32_t m = wave_min(gmin);
             if (c.lane == 0) sm.tmp[c.wave] = m;
             __syncthreads();
-            int32_t bm = c.lane < kW ? sm.tmp[c.lane] : INT32_MAX;
-#pragma unroll
-            for (int d = kW / 2; d > 0; d >>= 1) bm = min(bm, __shfl_xor(bm, d, kWave));
-            bm = lane_get(bm, 0);
+            int32_t bm = row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); });
+            bm = lane_get(bm, 15);
             key = (int64_t)INT32_MAX - bm;
         }
         const int64_t blk_fe = block_sum(fedges, sm);
