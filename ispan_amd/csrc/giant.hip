@@ -40,9 +40,8 @@ struct Smem {
     int32_t bi[kW];
     int ok;
     int32_t wbuf[kW][kWaveBuf];
-    int32_t wcnt[kW];       // block flushes: per-wave counts, offsets, block base
-    int32_t wbase[kW];
-    int32_t bbase;
+    int32_t wcnt[2][kW];    // block flushes: per-wave counts and bases of up to two counters
+    int32_t wbase[2][kW];
     int32_t hv[kW][kWave];  // per-wave staged hub vertices: id, first edge, degree
     int32_t hs[kW][kWave];
     int32_t hd[kW][kWave];
@@ -82,9 +81,12 @@ struct Red {
 // (and loads that tolerate stale values).
 //
 // Returns false (and flags an error) if a wait exceeds kSpinCap.
+//
+// USE: which of s0, s1, s2, key (bits 0-3) the barrier carries (the same in
+// every block); the others read back as 0.
 template <bool RED = false, bool ACQ = true>
 __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg, Red in = Red{0, 0, 0, 0},
-                                      Red* out = nullptr) {
+                                      Red* out = nullptr, int use = 15) {
     using ull = unsigned long long;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -106,10 +108,10 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
         if (atomicAdd(&gs->sub[grp].v, 1) == gsize - 1) {
             st_agent(&gs->sub[grp].v, 0);
             if (RED) {
-                const ull a0 = atomicExch(reinterpret_cast<ull*>(&ga.sum[0]), 0ull);
-                const ull a1 = atomicExch(reinterpret_cast<ull*>(&ga.sum[1]), 0ull);
-                const ull a2 = atomicExch(reinterpret_cast<ull*>(&ga.sum[2]), 0ull);
-                const ull ak = atomicExch(reinterpret_cast<ull*>(&ga.key), 0ull);
+                const ull a0 = (use & 1) ? atomicExch(reinterpret_cast<ull*>(&ga.sum[0]), 0ull) : 0ull;
+                const ull a1 = (use & 2) ? atomicExch(reinterpret_cast<ull*>(&ga.sum[1]), 0ull) : 0ull;
+                const ull a2 = (use & 4) ? atomicExch(reinterpret_cast<ull*>(&ga.sum[2]), 0ull) : 0ull;
+                const ull ak = (use & 8) ? atomicExch(reinterpret_cast<ull*>(&ga.key), 0ull) : 0ull;
                 if (a0) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[0]), a0);
                 if (a1) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[1]), a1);
                 if (a2) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[2]), a2);
@@ -120,10 +122,10 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
                 st_agent(&gs->top.v, 0);
                 if (RED) {
                     RLine& r = gs->res[(g0 + 1) & 1];
-                    st_agent(&r.sum[0], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[0]), 0ull));
-                    st_agent(&r.sum[1], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[1]), 0ull));
-                    st_agent(&r.sum[2], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[2]), 0ull));
-                    st_agent(&r.key, (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.key), 0ull));
+                    if (use & 1) st_agent(&r.sum[0], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[0]), 0ull));
+                    if (use & 2) st_agent(&r.sum[1], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[1]), 0ull));
+                    if (use & 4) st_agent(&r.sum[2], (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.sum[2]), 0ull));
+                    if (use & 8) st_agent(&r.key, (int64_t)atomicExch(reinterpret_cast<ull*>(&gs->tacc.key), 0ull));
                 }
                 last = true;
             }
@@ -146,10 +148,10 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
         else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (RED && ok) {
             const RLine& r = gs->res[(g0 + 1) & 1];
-            sm.rres[0] = load_agent(&r.sum[0]);
-            sm.rres[1] = load_agent(&r.sum[1]);
-            sm.rres[2] = load_agent(&r.sum[2]);
-            sm.rres[3] = load_agent(&r.key);
+            sm.rres[0] = (use & 1) ? load_agent(&r.sum[0]) : 0;
+            sm.rres[1] = (use & 2) ? load_agent(&r.sum[1]) : 0;
+            sm.rres[2] = (use & 4) ? load_agent(&r.sum[2]) : 0;
+            sm.rres[3] = (use & 8) ? load_agent(&r.key) : 0;
         }
         sm.ok = ok && load_agent(&gs->err.v) == 0;
         if (ok) dbg[G + blockIdx.x] += 1;  // barriers passed
@@ -230,21 +232,45 @@ __device__ __forceinline__ void hub_push(HubStage& hs, bool pred, int32_t v, int
 // per block: each wave's count goes through LDS, thread 0 reserves the
 // block's total.  (One allocation per wave would put up to 4096 atomics on
 // one counter at the end of every level.)  All threads call them.
-__device__ __forceinline__ int block_reserve(Ctx& c, int wave_count, int32_t* gcount) {
+// Room for every wave's entries in up to two global counters: lane 0 of
+// each wave posts its counts, lanes 0-15 of wave 0 (one DPP row) scan them,
+// one lane takes both counters' atomics back to back, and every wave reads
+// its bases.  Two block barriers.  sm.wbase is rewritten only after the next
+// call's first barrier, which every wave reaches after reading its bases.
+__device__ __forceinline__ void block_reserve2(Ctx& c, int n0, int32_t* count0, int n1, int32_t* count1, int& base0,
+                                               int& base1) {
+    static_assert(kW == 16, "the wave counts fill one DPP row");
     Smem& sm = c.sm;
-    if (c.lane == 0) sm.wcnt[c.wave] = wave_count;
-    __syncthreads();
-    if (c.tid == 0) {
-        int tot = 0;
-        for (int w = 0; w < kW; ++w) {
-            sm.wbase[w] = tot;
-            tot += sm.wcnt[w];
-        }
-        sm.bbase = tot ? atomicAdd(gcount, tot) : 0;
+    if (c.lane == 0) {
+        sm.wcnt[0][c.wave] = n0;
+        sm.wcnt[1][c.wave] = n1;
     }
     __syncthreads();
-    const int base = sm.bbase + sm.wbase[c.wave];
-    __syncthreads();  // sm.* free again
+    if (c.wave == 0) {
+        const int x = c.lane < kW ? sm.wcnt[0][c.lane] : 0, y = c.lane < kW ? sm.wcnt[1][c.lane] : 0;
+        const int xi = row_incl(x, 0, [](int p, int q) { return p + q; });
+        const int yi = row_incl(y, 0, [](int p, int q) { return p + q; });
+        const int xt = lane_get(xi, kW - 1), yt = lane_get(yi, kW - 1);
+        int xb = 0, yb = 0;
+        if (c.lane == 0) {
+            if (xt) xb = atomicAdd(count0, xt);
+            if (yt) yb = atomicAdd(count1, yt);
+        }
+        xb = lane_get(xb, 0);
+        yb = lane_get(yb, 0);
+        if (c.lane < kW) {
+            sm.wbase[0][c.lane] = xb + xi - x;
+            sm.wbase[1][c.lane] = yb + yi - y;
+        }
+    }
+    __syncthreads();
+    base0 = sm.wbase[0][c.wave];
+    base1 = sm.wbase[1][c.wave];
+}
+
+__device__ __forceinline__ int block_reserve(Ctx& c, int wave_count, int32_t* gcount) {
+    int base, unused;
+    block_reserve2(c, wave_count, gcount, 0, gcount, base, unused);
     return base;
 }
 
@@ -255,7 +281,10 @@ __device__ __forceinline__ void block_flush(Ctx& c, WaveQueue& wq, int32_t* gq, 
     wave_sync();
 }
 
-__device__ __forceinline__ void block_hub_flush(Ctx& c, HubStage& hs, HubItem* hout, int32_t* hn_out) {
+// block_flush of the queue and of the staged hub vertices (as work items)
+// with one reservation round.
+__device__ __forceinline__ void block_flush_both(Ctx& c, WaveQueue& wq, int32_t* gq, int32_t* gcount, HubStage& hs,
+                                                 HubItem* hout, int32_t* hn_out) {
     int32_t v = 0, s0 = 0, nch = 0;
     if (c.lane < hs.n) {
         v = hs.v[c.lane];
@@ -263,11 +292,15 @@ __device__ __forceinline__ void block_hub_flush(Ctx& c, HubStage& hs, HubItem* h
         nch = (hs.d[c.lane] + kGiantChunk - 1) / kGiantChunk;
     }
     const int incl = wave_incl_scan(nch);
-    const int base = block_reserve(c, lane_get(incl, kWave - 1), hn_out) + incl - nch;
+    int qb, hb;
+    block_reserve2(c, wq.n, gcount, lane_get(incl, kWave - 1), hn_out, qb, hb);
+    for (int i = c.lane; i < wq.n; i += kWave) st_agent(&gq[qb + i], wq.buf[i]);
+    hb += incl - nch;
     for (int k = 0; k < nch; ++k) {
-        st_agent(&hout[base + k].v, v);
-        st_agent(&hout[base + k].start, s0 + k * kGiantChunk);
+        st_agent(&hout[hb + k].v, v);
+        st_agent(&hout[hb + k].start, s0 + k * kGiantChunk);
     }
+    wq.n = 0;
     hs.n = 0;
     wave_sync();
 }
@@ -445,10 +478,9 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             seeded += ok;
             push_next(ok, r, beg, wq, a.q0, &gs->qn[dir][0].v, hs, a.h0, &gs->hn[dir][0].v, fedges);
         }
-        block_flush(c, wq, a.q0, &gs->qn[dir][0].v);
-        block_hub_flush(c, hs, a.h0, &gs->hn[dir][0].v);
+        block_flush_both(c, wq, a.q0, &gs->qn[dir][0].v, hs, a.h0, &gs->hn[dir][0].v);
         Red r;
-        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{block_sum(seeded, sm), 0, 0, 0}, &r)) return -1;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{block_sum(seeded, sm), 0, 0, 0}, &r, 1)) return -1;
         *seeded_out = r.s0;
     }
 
@@ -594,8 +626,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     }
                 }
             }
-            block_flush(c, wq, qout, qn_out);
-            block_hub_flush(c, hs, hout, hn_out);
+            block_flush_both(c, wq, qout, qn_out, hs, hout, hn_out);
         } else {
             // ---- bottom-up over the candidate bitmap, one wave per group of
             // kU consecutive 64-vertex words; only that wave writes those
@@ -771,19 +802,22 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             }
         }
         // ---- level totals: every block computes the same sums
-        const int64_t blk = block_sum(my_found, sm);
+        // one block reduction: finds (high 32 bits) + pushed out-degrees (low
+        // 32 bits, at most E < 2^31), and backward the smallest id reached
+        int64_t pk = wave_sum((my_found << 32) + fedges);
+        int32_t bm = BW ? wave_min(gmin) : INT32_MAX;
+        if (c.lane == 0) {
+            sm.red[c.wave] = pk;
+            sm.tmp[c.wave] = bm;
+        }
+        __syncthreads();
+        pk = lane_get(row_incl(sm.red[c.lane & 15], (int64_t)0, [](int64_t x, int64_t y) { return x + y; }), 15);
+        if (BW) bm = lane_get(row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); }), 15);
+        __syncthreads();
+        const int64_t blk = (int64_t)((uint64_t)pk >> 32);
         if (topdown) c.found_td += blk;
         else c.found_bu += blk;
-        int64_t key = 0;  // BW: INT32_MAX - (smallest id reached); 0 = none
-        if (BW) {
-            const int32_t m = wave_min(gmin);
-            if (c.lane == 0) sm.tmp[c.wave] = m;
-            __syncthreads();
-            int32_t bm = row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); });
-            bm = lane_get(bm, 15);
-            key = (int64_t)INT32_MAX - bm;
-        }
-        const int64_t blk_fe = block_sum(fedges, sm);
+        const int64_t key = BW ? (int64_t)INT32_MAX - bm : 0;  // BW: INT32_MAX - (smallest id reached)
         if (tr && c.tid == 0) {
             const unsigned long long wt = (unsigned long long)((int64_t)wall_clock64() - tl0);
             atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmax[dir][L]), wt);
@@ -794,10 +828,16 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         // candidate bitmap that is fixed during the traversal, and visited
         // bits only as a hint for the claim atomics: their barriers skip the
         // L2 invalidation (the acquire below catches up where needed).
+        // One sum carries the level's finds (high 32 bits) and the next level's
+        // exact work (low 32 bits: at most E < 2^31, so no carry); the
+        // adjacency count only feeds the level trace, and the smallest id only
+        // matters backward.
         Red r;
-        if (!gsync<true, false>(gs, c.G, sm, a.dbg, Red{blk, c.edges_td + c.edges_bu - edges0, blk_fe, key}, &r))
+        if (!gsync<true, false>(gs, c.G, sm, a.dbg,
+                                Red{pk, a.trace ? c.edges_td + c.edges_bu - edges0 : 0, 0, key}, &r,
+                                1 | (a.trace ? 2 : 0) | (BW ? 8 : 0)))
             return -1;
-        const int64_t found = r.s0;
+        const int64_t found = (int64_t)((uint64_t)r.s0 >> 32);
         if (c.b == 0 && c.tid == 0 && L < kLevelTrace) {
             gs->lv_ts[dir][L] = wall_clock64();
             gs->lv_found[dir][L] = found;
@@ -805,7 +845,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             gs->lv_mode[dir][L] = topdown ? 0 : 1;
         }
         if (BW) *gmin_out = (int32_t)((int64_t)INT32_MAX - r.key);
-        const int64_t next_edges = r.s2;  // top-down levels: exact work of the next level
+        const int64_t next_edges = r.s0 & 0xffffffffll;  // top-down levels: exact work of the next level
         visited += found;
         ++L;
         if (found == 0) {
@@ -844,8 +884,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next, fedges);
                 }
             }
-            block_flush(c, wq, q_next, qn_next);
-            block_hub_flush(c, hs, h_next, hn_next);
+            block_flush_both(c, wq, q_next, qn_next, hs, h_next, hn_next);
             mode = 2;
             if (!gsync(gs, c.G, sm, a.dbg)) return -1;
         }
@@ -915,7 +954,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
             __hip_atomic_store(&gs->fqn[(s + 1) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int64_t t = trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v);
         Red r;
-        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{t, 0, 0, 0}, &r)) return false;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{t, 0, 0, 0}, &r, 1)) return false;
         trim1 += r.s0;
         n = load_agent(&gs->fqn[s % 3].v);
         cur ^= 1;
@@ -950,7 +989,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
         }
         const int64_t b2 = block_sum(c2, c.sm), b3 = block_sum(c3, c.sm);
         Red r;
-        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{b2, b3, 0, 0}, &r)) return false;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{b2, b3, 0, 0}, &r, 3)) return false;
         p2 = r.s0;
         p3 = r.s1;
         // P11: one more trim-1 sweep
