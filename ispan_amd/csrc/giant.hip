@@ -39,6 +39,7 @@ struct Smem {
     int64_t rres[4];  // reduction result of the last barrier
     int32_t bi[kW];
     int ok;
+    int32_t nbar;  // barriers this block has arrived at
     int32_t wbuf[kW][kWaveBuf];
     int32_t wcnt[2][kW];    // block flushes: per-wave counts and bases of up to two counters
     int32_t wbase[2][kW];
@@ -91,8 +92,10 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        dbg[blockIdx.x] += 1;  // barriers arrived at
-        const int32_t g0 = load_agent(&gs->gen.v);
+        // gen starts at 0 in every launch and each barrier bumps it once, so
+        // at this block's nb-th barrier it reads nb - 1: no load needed
+        const int32_t nb = ++sm.nbar;
+        const int32_t g0 = nb - 1;
         const int grp = blockIdx.x % kBarGroups;
         const int ngrp = min(kBarGroups, G);
         const int gsize = (G - grp + kBarGroups - 1) / kBarGroups;
@@ -105,7 +108,12 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         bool last = false;
-        if (atomicAdd(&gs->sub[grp].v, 1) == gsize - 1) {
+        const int arrived = atomicAdd(&gs->sub[grp].v, 1);
+        // diagnostics, stored while the arrival is in flight: barriers arrived
+        // at, and passed (this block's previous one)
+        dbg[blockIdx.x] = nb;
+        dbg[G + blockIdx.x] = nb - 1;
+        if (arrived == gsize - 1) {
             st_agent(&gs->sub[grp].v, 0);
             if (RED) {
                 const ull a0 = (use & 1) ? atomicExch(reinterpret_cast<ull*>(&ga.sum[0]), 0ull) : 0ull;
@@ -154,7 +162,6 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
             sm.rres[3] = (use & 8) ? load_agent(&r.key) : 0;
         }
         sm.ok = ok && load_agent(&gs->err.v) == 0;
-        if (ok) dbg[G + blockIdx.x] += 1;  // barriers passed
     }
     __syncthreads();
     if (RED && out) *out = Red{sm.rres[0], sm.rres[1], sm.rres[2], sm.rres[3]};
@@ -791,15 +798,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             if (tr && c.tid == 0)
                 atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][2]),
                           (unsigned long long)((int64_t)wall_clock64() - tl0));
-            const int64_t bscan = block_sum(scanned, sm);
-            c.edges_bu += bscan;
-            if (tr && c.tid == 0) {
-                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][0]), (unsigned long long)bscan);
-                if (!BW && c.b < 1024 && gs->bu1_scan[c.b] == 0) {
-                    gs->bu1_scan[c.b] = bscan + 1;
-                    gs->bu1_work[c.b] = (int64_t)wall_clock64() - tl0;
-                }
-            }
+            fedges = scanned;  // bottom-up pushes nothing: the low half of the level sum counts the scan
         }
         // ---- level totals: every block computes the same sums
         // one block reduction: finds (high 32 bits) + pushed out-degrees (low
@@ -815,8 +814,20 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         if (BW) bm = lane_get(row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); }), 15);
         __syncthreads();
         const int64_t blk = (int64_t)((uint64_t)pk >> 32);
-        if (topdown) c.found_td += blk;
-        else c.found_bu += blk;
+        if (topdown) {
+            c.found_td += blk;
+        } else {
+            const int64_t bscan = pk & 0xffffffffll;
+            c.found_bu += blk;
+            c.edges_bu += bscan;
+            if (tr && c.tid == 0) {
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][0]), (unsigned long long)bscan);
+                if (!BW && c.b < 1024 && gs->bu1_scan[c.b] == 0) {
+                    gs->bu1_scan[c.b] = bscan + 1;
+                    gs->bu1_work[c.b] = (int64_t)wall_clock64() - tl0;
+                }
+            }
+        }
         const int64_t key = BW ? (int64_t)INT32_MAX - bm : 0;  // BW: INT32_MAX - (smallest id reached)
         if (tr && c.tid == 0) {
             const unsigned long long wt = (unsigned long long)((int64_t)wall_clock64() - tl0);
@@ -1049,6 +1060,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     if (threadIdx.x == 0) {
         a.dbg[blockIdx.x] = 0;
         a.dbg[gridDim.x + blockIdx.x] = 0;
+        sm.nbar = 0;
     }
     Ctx c{a, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, lane_id(), wave_id(), sm, 0, 0, 0, 0};
     GiantState* gs = a.gs;
