@@ -498,10 +498,6 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         HubItem* hout = (L & 1) ? a.h0 : a.h1;
         int32_t* qn_out = &gs->qn[dir][(L + 1) % 3].v;
         int32_t* hn_out = &gs->hn[dir][(L + 1) % 3].v;
-        if (c.b == 0 && c.tid == 0) {  // counters of level L+2: last read during level L-1
-            __hip_atomic_store(&gs->qn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gs->hn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
         const bool topdown = mode != 1;
         const int64_t edges0 = c.edges_td + c.edges_bu;  // thread 0's are the block's
         const bool tr = a.trace && L < kLevelTrace;
@@ -843,6 +839,13 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         // exact work (low 32 bits: at most E < 2^31, so no carry); the
         // adjacency count only feeds the level trace, and the smallest id only
         // matters backward.
+        // counters of level L+2: last read at the start of level L-1, first
+        // written in level L+1 (reset here, where the store overlaps the
+        // barrier's own atomics instead of delaying this level's first load)
+        if (c.b == 0 && c.tid == 0) {
+            __hip_atomic_store(&gs->qn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gs->hn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         Red r;
         if (!gsync<true, false>(gs, c.G, sm, a.dbg,
                                 Red{pk, a.trace ? c.edges_td + c.edges_bu - edges0 : 0, 0, key}, &r,
