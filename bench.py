@@ -365,7 +365,7 @@ def main(argv=None) -> int:
     if a.autotune:
         # the project's autotuner: fastest (alpha, beta) on this graph; the
         # labels cannot change, and the main line above keeps the defaults
-        cands = I.tuning_grid(alphas=(2, 4, 10, 30), betas=(50, 200, 800))
+        cands = I.tuning_grid(alphas=(2, 4, 10, 30), betas=(50, 200, 800, 5000, 100000))
         best, cms = S.autotune(cands, reps=3)
         S.synchronize()
         t0 = time.perf_counter()
