@@ -29,10 +29,9 @@ constexpr int kW = kGBlock / kWave;
 enum Slot { kPivP, kPivI, kSurv, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
 static_assert(kFoundBU < kGiantSlots, "slot count");
 
-constexpr int kTileMax = kGBlock * 1;  // queue entries per top-down tile, at most
 struct Smem {
-    int32_t off[kTileMax];
-    int32_t start[kTileMax];
+    int32_t off[kGBlock];    // top-down tile: exclusive degree prefix, first edge
+    int32_t start[kGBlock];
     int tmp[kW];
     int64_t red[kW];
     int64_t red2[kW];
@@ -435,11 +434,8 @@ __device__ __forceinline__ void push_found_n(const bool (&claim)[K], const int32
     }
 }
 
-// Top-down: queue entries per thread per tile, and edges per lane per step
-// (their load chains overlap).
-constexpr int kTE = 1;
+// Top-down: edges per lane per step (their load chains overlap).
 constexpr int kTK = 2;
-constexpr int kTile = kGBlock * kTE;
 
 // One direction of the traversal.  Returns the number of vertices claimed
 // (the roots excluded), or -1 after a failed barrier.  BW: claims are limited
@@ -526,38 +522,22 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     nh = 0;
                 }
             }
-            static_assert(kTile <= kTileMax && (kTile & (kTile - 1)) == 0, "tile size");
-            const int64_t ntiles = (n + kTile - 1) / kTile;
+            static_assert((kGBlock & (kGBlock - 1)) == 0, "tile size");
+            const int64_t ntiles = (n + kGBlock - 1) / kGBlock;
             for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
                 if (u < ntiles) {
-                    // thread tid owns the tile's entries tid * kTE + t
-                    const int64_t i0 = u * kTile + (int64_t)c.tid * kTE;
-                    int32_t qv[kTE], st[kTE], dg[kTE];
-#pragma unroll
-                    for (int t = 0; t < kTE; ++t) qv[t] = i0 + t < n ? load_agent(&qin[i0 + t]) : -1;
-#pragma unroll
-                    for (int t = 0; t < kTE; ++t) {
-                        st[t] = 0;
-                        dg[t] = 0;
-                        if (qv[t] >= 0) {
-                            st[t] = beg[qv[t]];
-                            dg[t] = beg[qv[t] + 1];
-                        }
-                    }
-                    int sum = 0;
-#pragma unroll
-                    for (int t = 0; t < kTE; ++t) {
-                        dg[t] -= st[t];
-                        sum += dg[t];
+                    // a tile: one queue entry per thread
+                    const int64_t i = u * kGBlock + c.tid;
+                    int32_t st = 0, dg = 0;
+                    if (i < n) {
+                        const int32_t v = load_agent(&qin[i]);
+                        st = beg[v];
+                        dg = beg[v + 1] - st;
                     }
                     int excl, total;
-                    block_excl_scan<kW>(sum, excl, total, sm.tmp);
-#pragma unroll
-                    for (int t = 0; t < kTE; ++t) {
-                        sm.off[c.tid * kTE + t] = excl;
-                        sm.start[c.tid * kTE + t] = st[t];
-                        excl += dg[t];
-                    }
+                    block_excl_scan<kW>(dg, excl, total, sm.tmp);
+                    sm.off[c.tid] = excl;
+                    sm.start[c.tid] = st;
                     if (c.tid == 0) c.edges_td += total;
                     __syncthreads();
                     for (int eb = 0; eb < total; eb += kGBlock * kTK) {
@@ -571,7 +551,7 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                             pos[k] = 0;
                         }
 #pragma unroll
-                        for (int sp = kTile / 2; sp > 0; sp >>= 1) {
+                        for (int sp = kGBlock / 2; sp > 0; sp >>= 1) {
 #pragma unroll
                             for (int k = 0; k < kTK; ++k)
                                 if (sm.off[pos[k] + sp] <= e[k]) pos[k] += sp;
