@@ -523,13 +523,18 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                 }
             }
             static_assert((kGBlock & (kGBlock - 1)) == 0, "tile size");
-            const int64_t ntiles = (n + kGBlock - 1) / kGBlock;
+            // entries per tile: a power of two from 64 to kGBlock, the
+            // smallest that needs no more tiles than blocks, so a small
+            // frontier is spread over many blocks
+            int tile = kGBlock;
+            while (tile > kWave && (int64_t)(tile / 2) * c.G >= n) tile /= 2;
+            const int64_t ntiles = (n + tile - 1) / tile;
             for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
                 if (u < ntiles) {
-                    // a tile: one queue entry per thread
-                    const int64_t i = u * kGBlock + c.tid;
+                    // a tile: queue entries on threads 0..tile-1
+                    const int64_t i = u * tile + c.tid;
                     int32_t st = 0, dg = 0;
-                    if (i < n) {
+                    if (c.tid < tile && i < n) {
                         const int32_t v = load_agent(&qin[i]);
                         st = beg[v];
                         dg = beg[v + 1] - st;
