@@ -434,8 +434,10 @@ __device__ __forceinline__ void push_found_n(const bool (&claim)[K], const int32
     }
 }
 
-// Top-down: edges per lane per step (their load chains overlap).
-constexpr int kTK = 2;
+// Top-down: edges per lane per step.  One measured fastest: two let their
+// load chains overlap, but the registers they hold push the kernel into
+// spills on other paths.
+constexpr int kTK = 1;
 
 // One direction of the traversal.  Returns the number of vertices claimed
 // (the roots excluded), or -1 after a failed barrier.  BW: claims are limited
