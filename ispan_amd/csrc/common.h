@@ -159,8 +159,10 @@ struct GiantArgs {
 
 constexpr int kGiantSlots = 16;
 constexpr int kGiantHubDeg = 512;   // degree from which a claimed vertex is split into work items
-constexpr int kGiantChunk = 2048;   // edges per work item
-inline int64_t giant_hub_capacity(int64_t E) { return 5 * (E / kGiantChunk) + 64; }
+constexpr int kGiantChunk = 8192;   // edges per work item
+// A vertex of degree d >= kGiantHubDeg becomes ceil(d / kGiantChunk) <=
+// d / kGiantChunk + 1 items, and at most E / kGiantHubDeg vertices qualify.
+inline int64_t giant_hub_capacity(int64_t E) { return E / kGiantChunk + E / kGiantHubDeg + 64; }
 
 int giant_max_blocks_per_cu();
 // GiantArgs::bu_mult for a grid of `grid` blocks
