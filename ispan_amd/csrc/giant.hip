@@ -1071,7 +1071,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     // a wave takes kUA consecutive words at a time and issues all their offset
     // loads before it uses any of them; lane i loads beg[v] once and takes
     // beg[v + 1] from lane i + 1 (lane 63 from the next word's lane 0)
-    constexpr int kUA = 8;
+    constexpr int kUA = 16;
     // lane u < kUA holds the previous step's word w0 + u until the next
     // step's loads are out: a load's wait also waits for earlier stores
     int64_t pword = -1;
