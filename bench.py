@@ -305,8 +305,8 @@ def main(argv=None) -> int:
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration, 5)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build runs the SCC pipeline on a single MI355X "
-              "(multi-GPU sharding is design-only); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the SCC pipeline runs on a single MI355X "
+              "(multi-GPU sharding is not implemented); use --gpus 1", file=sys.stderr)
         return 2
 
     import ispan_amd as I

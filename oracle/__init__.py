@@ -1,8 +1,8 @@
 """TEST INFRASTRUCTURE -- the oracle.  Not part of the product.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker (or as the timed
-CPU baseline).  The product library never loads it.
+``bench.py`` may import this package, and only to check results against (or as
+the timed CPU baseline).  The product library never loads it.
 
 ctypes wrappers over ``oracle/liboracle.so`` (built by ``oracle/Makefile``):
 

@@ -2,8 +2,8 @@
 // TEST INFRASTRUCTURE -- THE ORACLE.  Not part of the product.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
-// this library, and only as the checker (or as the timed CPU baseline).  The
-// product (libispan_hip.so) never links or calls it.
+// this library, and only to check results against (or as the timed CPU
+// baseline).  The product (libispan_hip.so) never links or calls it.
 //
 // Contents (all exported with a C ABI for ctypes):
 //   * orc_rmat_edges / orc_grid_edges : seeded counter-hash graph generators
