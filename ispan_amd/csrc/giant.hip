@@ -621,8 +621,9 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             // ---- bottom-up over the candidate bitmap, one wave per group of
             // kU consecutive 64-vertex words; only that wave writes those
             // words in this level.  Each stage issues its kU loads (first
-            // in-neighbours, their visited bits) before any result is used.  Found vertices also go to the next queue, so
-            // a switch back to top-down needs no extra pass.
+            // in-neighbours, their visited bits) before any result is used.
+            // Nothing is queued: each word's visited bits before the level go
+            // to `snap`, and a switch back to top-down gathers vis & ~snap.
             constexpr int kU = 8;       // dense words per group
             constexpr int kSparse = 8;  // at most this many candidates: a lane's own work
             constexpr int kS = 4;       // sparse candidates per lane per step
