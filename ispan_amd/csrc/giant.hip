@@ -888,7 +888,10 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
             }
             block_flush_both(c, wq, q_next, qn_next, hs, h_next, hn_next);
             mode = 2;
-            if (!gsync(gs, c.G, sm, a.dbg)) return -1;
+            // the top-down level reads the queue coherently and visited bits
+            // only as hints: no L2 invalidation (the one above covered the
+            // gather's exact reads)
+            if (!gsync<false, false>(gs, c.G, sm, a.dbg)) return -1;
         }
     }
     *levels_out = L;
