@@ -212,6 +212,28 @@ def test_stats_consistency(solver):
     assert sum(kp.found[:3]) == st.fw_size + st.giant_size - 2  # every vertex but the two roots is claimed once
 
 
+@pytest.mark.parametrize("engine", [0, 1])
+@pytest.mark.parametrize("params", [dict(alpha=1), dict()], ids=["topdown", "default"])
+def test_hub_split_at_a_later_level(solver, engine, params):
+    # the pivot 0 reaches the hub 1 (and 1000 others) at level 0; the hub's
+    # list (20 000 vertices that point back to 0, then 100 sinks) is split
+    # into several work items, the last one partial, and expanded at level 1;
+    # alpha = 1 keeps the forward traversal top-down
+    A, B, S = range(2, 1002), range(1002, 21002), range(21002, 21102)
+    edges = [(0, 1)] + [(0, a) for a in A] + [(a, 0) for a in A]
+    edges += [(1, b) for b in B] + [(b, 0) for b in B] + [(1, s) for s in S]
+    g = O.graph_from_edges(21102, edges)
+    st = check(solver, g, engine=engine, timing=2, **params)
+    assert st.pivot == 0 and st.giant_size == 21002 and st.fw_size == 21102 - 100
+    if params:
+        solver.solve_stats(st)
+        kp = solver.kernel_profile()
+        # forward alone claims the hub and A at level 0, then B from the hub's
+        # items, all top-down (the backward traversal may switch to bottom-up)
+        assert kp.found[0] + kp.found[1] >= 1001 + 20000
+        assert sum(kp.found[:3]) == st.fw_size + st.giant_size - 2
+
+
 # ------------------------------------------------ device graph producers
 
 @pytest.mark.parametrize("key,d", G.hash_cases(), ids=[k for k, _ in G.hash_cases()])
