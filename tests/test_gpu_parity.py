@@ -3,6 +3,7 @@ equal the oracle's -- Tarjan's labelling, the golden fixtures, and the
 sha256 of Tarjan's labels for the large seeded graphs.  Integer work: the
 bar is bit-exact equality."""
 import ctypes as C
+import functools
 import os
 import subprocess
 
@@ -210,6 +211,23 @@ def test_stats_consistency(solver):
     kp = solver.kernel_profile()
     assert kp.launches[3] == 1 and kp.found[3] == st.fq0
     assert sum(kp.found[:3]) == st.fw_size + st.giant_size - 2  # every vertex but the two roots is claimed once
+
+
+@functools.lru_cache(maxsize=1)
+def uniform_digraph(V, E, seed):
+    rng = np.random.default_rng(seed)
+    return O.graph_from_edges(V, np.stack([rng.integers(0, V, E), rng.integers(0, V, E)], axis=1))
+
+
+@pytest.mark.parametrize("engine", [0, 1])
+def test_large_top_down_frontiers(solver, engine):
+    # uniform random digraph, 10^6 vertices, out-degree 4: with alpha = 1 the
+    # forward frontier grows 1 -> 11 -> ... -> 42 K -> 148 K -> 355 K -> 333 K
+    # top-down before the switch, so its levels stage queue tiles of 64, 256
+    # and 1024 entries
+    g = uniform_digraph(1_000_000, 4_000_000, 11)
+    st = check(solver, g, engine=engine, alpha=1)
+    assert st.fw_levels >= 10 and st.giant_size > 900_000
 
 
 @pytest.mark.parametrize("engine", [0, 1])
