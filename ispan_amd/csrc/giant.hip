@@ -7,9 +7,10 @@
 // Every block runs the same control flow: after each barrier all blocks sum
 // the same per-block partial counts and take the same top-down/bottom-up
 // decision, so every block passes the same barriers.  Data written by other
-// blocks before a barrier is read with plain loads after it (every block
-// invalidates its caches when it passes a barrier); device-coherent loads are
-// kept for what other blocks change during the same phase.
+// blocks before a barrier is read with plain loads after a barrier that
+// invalidates the caches, and with device-coherent loads after one that does
+// not; device-coherent loads are also kept for what other blocks change
+// during the same phase.
 #include "common.h"
 #include "device_util.h"
 
@@ -60,15 +61,29 @@ struct Red {
     int64_t s0, s1, s2, key;
 };
 
+// Waits until every vector-memory access this wave has issued (loads,
+// stores, atomics) has completed: s_waitcnt vmcnt(0), with no cache
+// maintenance.  An sc1 store or a device atomic has completed once it is
+// visible at device scope, so after this wait the wave's device-coherent
+// writes are visible to every other XCD.  (A release fence does not give
+// this: at workgroup scope it emits no wait at all, since the block's waves
+// share one L1; at agent scope it also writes back the whole L2.)
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0), others unchanged
+
 // Two-level grid barrier: blocks arrive on one of 32 counters, the last of
 // each group arrives on the top counter, the last overall bumps `gen`.
 //
 // Memory protocol: everything a block writes for other blocks is written
-// device-coherently (atomics or st_agent), so arriving needs no cache
-// writeback, only that the block's own accesses have completed (a
-// workgroup-scope fence waits for them).  After `gen` moves, each block
-// invalidates its L1 and its XCC's L2 (device-scope acquire), so plain loads
-// after the barrier see what every block wrote before it.
+// device-coherently (atomics or st_agent), so arriving needs no L2
+// writeback.  It does need every one of those writes to have completed
+// before the arrival, and they are issued by all 16 waves while only thread
+// 0 arrives: so every wave waits for its own accesses (wait_vm) before the
+// block barrier, and thread 0 waits again before each of its arrival
+// atomics (its reduction atomics and counter resets precede them).  The
+// last block's reduction results and counter resets likewise complete
+// before it bumps `gen`.  A block that sees `gen` move therefore sees every
+// write made before the barrier, once it invalidates its L1 and its XCC's
+// L2 (device-scope acquire) or reads the data with device-coherent loads.
 //
 // With RED, each block's `in` is folded in on the way: into its group's
 // accumulator before arriving, the group total into the top accumulator by
@@ -88,6 +103,7 @@ template <bool RED = false, bool ACQ = true>
 __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* dbg, Red in = Red{0, 0, 0, 0},
                                       Red* out = nullptr, int use = 15) {
     using ull = unsigned long long;
+    wait_vm();  // this wave's writes for other blocks have completed
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
@@ -105,7 +121,7 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
             if (in.s2) atomicAdd(reinterpret_cast<ull*>(&ga.sum[2]), (ull)in.s2);
             if (in.key) atomicMax(reinterpret_cast<ull*>(&ga.key), (ull)in.key);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        wait_vm();  // the group accumulator holds this block's values before it arrives
         bool last = false;
         const int arrived = atomicAdd(&gs->sub[grp].v, 1);
         // diagnostics, stored while the arrival is in flight: barriers arrived
@@ -124,7 +140,7 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
                 if (a2) atomicAdd(reinterpret_cast<ull*>(&gs->tacc.sum[2]), a2);
                 if (ak) atomicMax(reinterpret_cast<ull*>(&gs->tacc.key), ak);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            wait_vm();  // the group counter's reset and the group's totals precede the top arrival
             if (atomicAdd(&gs->top.v, 1) == ngrp - 1) {
                 st_agent(&gs->top.v, 0);
                 if (RED) {
@@ -138,7 +154,7 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
             }
         }
         if (last) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            wait_vm();  // the top counter's reset and the results precede the new generation
             atomicAdd(&gs->gen.v, 1);
         } else {
             int spins = 0;
