@@ -43,6 +43,7 @@ extern "C" {
 #define ISPAN_ERR_BUDGET (-9)     /* server: the HBM budget cannot admit it      */
 #define ISPAN_ERR_TIMEOUT (-10)   /* server: the wait timed out                  */
 #define ISPAN_ERR_NOTFOUND (-11)  /* server: unknown graph or request id         */
+#define ISPAN_ERR_SCRATCH (-12)   /* too large for ispan_scc_detection_noalloc   */
 
 /* Tuning knobs.  alpha/beta/gamma/theta keep the meaning of the `ispan`
  * argv (src/scc_main.cpp:20-25; used at src/fw_bw.h:581,599 and
@@ -161,6 +162,27 @@ int ispan_kernel_profile_get(const ispan_ctx* ctx, ispan_kernel_profile* out);
  * scc_detection + the label hook of get_scc_result).  labels: int32[V]. */
 int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32_t* fw_adj, const int32_t* bw_beg,
                         const int32_t* bw_adj, const ispan_params* params, int32_t* labels, ispan_stats* stats);
+
+/* Allocation-free variant of ispan_scc_detection, on device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  It calls no hipMalloc/hipFree, creates no stream or event and
+ * allocates no host memory on success: every buffer of the solve comes from
+ * one fixed device array of 64 MiB that the library holds on each GPU it runs
+ * on until the process ends.  The solve state takes about 69 bytes per vertex
+ * plus 0.05 bytes per edge plus ~0.4 MB, so graphs of up to ~900,000 vertices
+ * fit (R-MAT scale 19, edge factor 16 does; scale 20 does not); a larger one
+ * returns ISPAN_ERR_SCRATCH before any device work.  Inputs: the CSR pair in
+ * device memory (as ispan_graph_bind_device); output: d_labels (int32[V],
+ * device).  Runs on the current HIP device, on hip_stream (NULL: the null
+ * stream), and returns when the labels are in d_labels.  params->timing is
+ * ignored (stats->ms[3] is still the wall time).  Every call on one GPU uses
+ * that GPU's one array, which keeps the last call's contents (each solve
+ * clears what it reads): calls that overlap on one GPU race, so callers must
+ * serialise them per device. */
+int ispan_scc_detection_noalloc(int32_t V, int64_t E, const int32_t* d_fw_beg, const int32_t* d_fw_adj,
+                                const int32_t* d_bw_beg, const int32_t* d_bw_adj, const ispan_params* params,
+                                int32_t* d_labels, void* hip_stream, ispan_stats* stats);
 
 /* ---- batched solves ----------------------------------------------------
  * Many graphs in one pass of the pipeline over their disjoint union.  There

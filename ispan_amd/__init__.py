@@ -28,7 +28,7 @@ CLI_PATH = os.path.join(PKG, "bin", "ispan")
 ISPAN_OK = 0
 ERRORS = {-1: "ISPAN_ERR_ARG", -2: "ISPAN_ERR_CSR", -3: "ISPAN_ERR_NOMEM", -4: "ISPAN_ERR_HIP",
           -5: "ISPAN_ERR_ITER", -6: "ISPAN_ERR_UNLABELLED", -7: "ISPAN_ERR_IO", -8: "ISPAN_ERR_NOGRAPH",
-          -9: "ISPAN_ERR_BUDGET", -10: "ISPAN_ERR_TIMEOUT", -11: "ISPAN_ERR_NOTFOUND"}
+          -9: "ISPAN_ERR_BUDGET", -10: "ISPAN_ERR_TIMEOUT", -11: "ISPAN_ERR_NOTFOUND", -12: "ISPAN_ERR_SCRATCH"}
 
 # Every symbol include/ispan_hip.h declares.
 EXPORTS = [
@@ -41,7 +41,7 @@ EXPORTS = [
     "ispan_batch_labels_download", "ispan_batch_labels_device", "ispan_server_create", "ispan_server_destroy",
     "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit", "ispan_server_submit_many",
     "ispan_server_wait", "ispan_server_wait_device", "ispan_server_release", "ispan_server_stats_get",
-    "ispan_server_reset_stats", "ispan_server_last_error",
+    "ispan_server_reset_stats", "ispan_server_last_error", "ispan_scc_detection_noalloc",
 ]
 
 
@@ -160,6 +160,8 @@ def lib():
         "ispan_solve_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
         "ispan_scc_detection": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
                                           C.POINTER(Stats)]),
+        "ispan_scc_detection_noalloc": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
+                                                  _vp, C.POINTER(Stats)]),
         "ispan_graph_load": (C.c_int, [C.c_char_p] * 4 + [C.POINTER(HostGraphC)]),
         "ispan_graph_free": (None, [C.POINTER(HostGraphC)]),
         "ispan_csr_check": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]),

@@ -15,7 +15,9 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ispan_hip.h"
@@ -856,6 +858,7 @@ const char* ispan_strerror(int code) {
         case ISPAN_ERR_BUDGET: return "HBM budget exceeded";
         case ISPAN_ERR_TIMEOUT: return "timed out";
         case ISPAN_ERR_NOTFOUND: return "unknown id";
+        case ISPAN_ERR_SCRATCH: return "graph too large for the allocation-free solve's fixed device scratch";
         default: return "unknown error";
     }
 }
@@ -1419,6 +1422,147 @@ int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32
     if (!rc) rc = ispan_labels_download(ctx, labels);
     ispan_ctx_destroy(ctx);
     return rc;
+}
+
+// ---- allocation-free one-shot solve ---------------------------------------
+// Every buffer of ispan_scc_detection_noalloc's solve lives in this array:
+// one copy per GPU, held until the process ends.  In host code the name is
+// a host-side stand-in, never a device address: the device address comes
+// from hipGetSymbolAddress, taken once per device and kept below.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 16;
+
+struct ScratchDevice {
+    bool ready = false;
+    unsigned char* ptr = nullptr;  // this GPU's `scratch` (device address)
+    int cu = 0, grid = 0, coop_grid = 0;
+    double clock_khz = 0.0;
+    Ctl h_ctl;                     // host copies of the control block and the
+    GiantState h_gstate;           // giant kernel's state, read after each wait
+};
+std::mutex g_scratch_mu;
+ScratchDevice g_scratch[kMaxScratchDevices];
+
+// Set-up of device `dev` (current): the address of its copy of `scratch`
+// and its launch geometry, on its first call.
+int scratch_device(int dev, ScratchDevice** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchDevice& sd = g_scratch[dev];
+    if (!sd.ready) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return ISPAN_ERR_HIP;
+        int cu = 0, coop = 0, rate = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return ISPAN_ERR_HIP;
+        (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+        (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev);
+        sd.cu = cu;
+        sd.grid = std::max(cu, 1) * 8;  // as ispan_ctx_create
+        sd.coop_grid = (coop ? std::min(giant_max_blocks_per_cu(), 8) : 0) * cu;
+        sd.clock_khz = rate;
+        sd.ptr = static_cast<unsigned char*>(p);
+        sd.ready = true;
+    }
+    *out = &sd;
+    return ISPAN_OK;
+}
+
+// Places the buffers ensure_buffers would allocate (same sizes), the control
+// block, the giant kernel's state and the first-neighbour arrays of bind
+// one after another from base, 256-byte aligned; with base == nullptr only
+// sizes them.  Returns the bytes used.
+size_t carve_solve_state(ispan_ctx* c, unsigned char* base, int32_t V, int64_t E, size_t sort_bytes) {
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t bytes) {
+        using T = std::remove_reference_t<decltype(p)>;
+        p = base ? reinterpret_cast<T>(base + off) : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+    };
+    const size_t nv = (size_t)std::max(V, 1) + 1;
+    const size_t nw = (size_t)((V + 63) / 64) + 1;
+    take(c->lab, nv * 4);
+    take(c->act, nw * 8);
+    take(c->fvis, nw * 8);
+    take(c->bvis, nw * 8);
+    for (int i = 0; i < 2; ++i) {
+        take(c->fq[i], nv * 4);
+        take(c->q[i], nv * 4);
+        take(c->hub[i], (size_t)hub_capacity(E) * sizeof(HubItem));
+    }
+    take(c->parent, nv * 4);
+    take(c->stamp, nv * 4);
+    take(c->keys, nv * 8);
+    take(c->keys_alt, nv * 8);
+    take(c->starts, nv * 4);
+    take(c->qbuf, nv * 4);
+    take(c->part, (size_t)c->grid * sizeof(PivotPart));
+    take(c->snap, nw * 8);
+    for (int i = 0; i < 2; ++i) take(c->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem));
+    take(c->gpart, (size_t)kGiantSlots * std::max(c->coop_grid, 1) * 8);
+    take(c->gdbg, (size_t)2 * std::max(c->coop_grid, 1) * 4);
+    unsigned char* st = nullptr;
+    take(st, sort_bytes + 256);
+    c->sort_tmp = st;
+    c->sort_tmp_bytes = sort_bytes + 256;
+    take(c->ctl, sizeof(Ctl));
+    take(c->gstate, sizeof(GiantState));
+    take(c->fw_first, (size_t)std::max(V, 1) * 4);
+    take(c->bw_first, (size_t)std::max(V, 1) * 4);
+    return off;
+}
+
+}  // namespace
+
+int ispan_scc_detection_noalloc(int32_t V, int64_t E, const int32_t* d_fw_beg, const int32_t* d_fw_adj,
+                                const int32_t* d_bw_beg, const int32_t* d_bw_adj, const ispan_params* params,
+                                int32_t* d_labels, void* hip_stream, ispan_stats* stats) {
+    if (V < 0 || V == INT32_MAX || E < 0 || E > INT32_MAX) return ISPAN_ERR_ARG;
+    if (V > 0 && (!d_fw_beg || !d_bw_beg || !d_labels)) return ISPAN_ERR_ARG;
+    if (E > 0 && (!d_fw_adj || !d_bw_adj)) return ISPAN_ERR_ARG;
+    ispan_params P;
+    ispan_default_params(&P);
+    if (params) P = *params;
+    if (P.alpha <= 0 || P.beta <= 0 || P.trim_times < 0) return ISPAN_ERR_ARG;
+    P.timing = 0;  // phase timing would need events created
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) return ISPAN_ERR_HIP;
+    ScratchDevice* sd = nullptr;
+    int rc = scratch_device(dev, &sd);
+    if (rc) return rc;
+    ispan_ctx c;
+    c.device = dev;
+    c.stream = static_cast<hipStream_t>(hip_stream);
+    c.cu = sd->cu;
+    c.grid = sd->grid;
+    c.coop_grid = sd->coop_grid;
+    c.clock_khz = sd->clock_khz;
+    size_t sort_bytes = 0;
+    if (rocprim::radix_sort_keys(nullptr, sort_bytes, static_cast<uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+                                 (size_t)std::max(V, 1) + 1, 0, 64, c.stream) != hipSuccess)
+        return ISPAN_ERR_HIP;
+    if (carve_solve_state(&c, nullptr, V, E, sort_bytes) > kScratchBytes) return ISPAN_ERR_SCRATCH;
+    carve_solve_state(&c, sd->ptr, V, E, sort_bytes);
+    c.cap_V = V;  // the buffers are in place: ensure_buffers allocates nothing
+    c.cap_E = E;
+    c.h_ctl = &sd->h_ctl;
+    c.h_gstate = &sd->h_gstate;
+    c.V = V;
+    c.E = E;
+    c.nwords = ((int64_t)V + 63) / 64;
+    c.fwb = d_fw_beg;
+    c.fwa = d_fw_adj;
+    c.bwb = d_bw_beg;
+    c.bwa = d_bw_adj;
+    if ((rc = check_bound(&c, &P))) return rc;
+    if (V > 0) {
+        if (launch_first(c.fwb, c.fwa, V, c.fw_first, c.grid, c.stream) != hipSuccess ||
+            launch_first(c.bwb, c.bwa, V, c.bw_first, c.grid, c.stream) != hipSuccess)
+            return ISPAN_ERR_HIP;
+    }
+    return solve_impl(&c, &P, d_labels, stats);
 }
 
 int ispan_graph_load(const char* fw_beg_file, const char* fw_adj_file, const char* bw_beg_file,
