@@ -321,6 +321,13 @@ typedef struct ispan_summary {
     int64_t total;
 } ispan_summary;
 int ispan_summarize(int32_t V, const int32_t* labels, ispan_summary* out);
+/* SCC-size histogram of the last ispan_solve's labels, computed on the
+ * device: bins[k] (host, n_bins <= 64 entries) = number of SCCs with size in
+ * [2^k, 2^(k+1)); the last bin also counts every larger size.  summary (may
+ * be NULL) receives what ispan_summarize gives for the same labels.  Extends
+ * get_scc_result (src/load_print.cpp:24-60), which tallies the sizes of the
+ * SCCs it cannot classify in a std::map but prints only count classes. */
+int ispan_scc_histogram(ispan_ctx* ctx, int64_t* bins, int32_t n_bins, ispan_summary* summary);
 
 /* ---- synthetic inputs built on the device ------------------------------ */
 

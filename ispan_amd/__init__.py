@@ -42,6 +42,7 @@ EXPORTS = [
     "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit", "ispan_server_submit_many",
     "ispan_server_wait", "ispan_server_wait_device", "ispan_server_release", "ispan_server_stats_get",
     "ispan_server_reset_stats", "ispan_server_last_error", "ispan_scc_detection_noalloc",
+    "ispan_scc_histogram",
 ]
 
 
@@ -160,6 +161,7 @@ def lib():
         "ispan_solve_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
         "ispan_scc_detection": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
                                           C.POINTER(Stats)]),
+        "ispan_scc_histogram": (C.c_int, [_vp, _vp, C.c_int32, C.POINTER(Summary)]),
         "ispan_scc_detection_noalloc": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
                                                   _vp, C.POINTER(Stats)]),
         "ispan_graph_load": (C.c_int, [C.c_char_p] * 4 + [C.POINTER(HostGraphC)]),
@@ -384,6 +386,15 @@ class Solver:
 
     def labels_device_ptr(self) -> int:
         return lib().ispan_labels_device(self._ctx) or 0
+
+    def scc_histogram(self, n_bins: int = 64):
+        """SCC-size histogram of the last solve (computed on the device):
+        bins[k] counts SCCs of size in [2^k, 2^(k+1)), plus the summary dict
+        of get_scc_result."""
+        bins = np.zeros(n_bins, np.int64)
+        sm = Summary()
+        _check(lib().ispan_scc_histogram(self._ctx, _ptr(bins) if n_bins else None, n_bins, C.byref(sm)), self._ctx)
+        return bins, {k: int(getattr(sm, k)) for k, _ in Summary._fields_}
 
     # ---- batched solves (many graphs, one pass over their disjoint union)
     def batch_bind(self, batch: "DeviceBatch", max_group_edges: int = 0, params: Params | None = None):

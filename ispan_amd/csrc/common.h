@@ -253,6 +253,10 @@ hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, const uint64_t
                                    hipStream_t s);
 hipError_t launch_stats(const int32_t* fwb, const int32_t* bwb, const int32_t* lab, const uint64_t* fvis,
                         const uint64_t* bvis, int32_t V, Ctl* ctl, int grid, hipStream_t s);
+// SCC sizes of canonical labels into size[V] (cleared first), then
+// out[kHistSlots]: 64 log2 bins, largest, counts of sizes 1-3, SCC count.
+constexpr int kHistSlots = 69;
+hipError_t launch_scc_histogram(const int32_t* lab, int32_t V, int32_t* size, int64_t* out, int grid, hipStream_t s);
 hipError_t launch_check_csr(const int32_t* beg, const int32_t* adj, int32_t V, int64_t E, Ctl* ctl, int grid,
                             hipStream_t s);
 

@@ -10,9 +10,15 @@
 // result block counts SCCs by size from the canonical labels (the reference
 // counts by the phase that found them and adds one phantom entry).  With
 // ISPAN_LABELS_OUT=<file> the canonical labels of the last run are written as
-// int32[V].
+// int32[V].  An optional 11th argument `--histogram` (not in the reference)
+// adds, after each result block, the SCC count per size range:
+//   SCC size histogram:
+//   size 1, <n>
+//   size 2-3, <n>
+//   size 4-7, <n>      (one line per non-empty power-of-two range)
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <sys/time.h>
 
 #include <vector>
@@ -33,7 +39,8 @@ static int die(const char* what, int rc, ispan_ctx* ctx) {
 
 int main(int argc, char** argv) {
     printf("args = %d\n", argc);
-    if (argc != 11) {
+    const bool histogram = argc == 12 && strcmp(argv[11], "--histogram") == 0;
+    if (argc != 11 && !histogram) {
         printf("Usage: ./scc_cpu <fw_beg_file> <fw_csr_file> <bw_beg_file> <bw_csr_file> <thread_count> <alpha> "
                "<beta> <gamma> <theta> <run_times>\n");
         return 255;
@@ -81,6 +88,16 @@ int main(int argc, char** argv) {
                "%lld\ntotal, %lld\n",
                (long long)sm.largest, (long long)sm.size_1, (long long)sm.size_2, (long long)sm.size_3,
                (long long)sm.others, (long long)sm.total);
+        if (histogram) {
+            int64_t bins[32];
+            if ((rc = ispan_scc_histogram(ctx, bins, 32, nullptr))) return die("histogram", rc, ctx);
+            printf("\nSCC size histogram:\n");
+            for (int k = 0; k < 32; ++k) {
+                if (!bins[k]) continue;
+                if (k == 0) printf("size 1, %lld\n", (long long)bins[k]);
+                else printf("size %lld-%lld, %lld\n", 1ll << k, (2ll << k) - 1, (long long)bins[k]);
+            }
+        }
     }
     const char* out = getenv("ISPAN_LABELS_OUT");
     if (out && *out && run_times > 0) {

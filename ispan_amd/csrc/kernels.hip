@@ -747,6 +747,74 @@ __global__ __launch_bounds__(kBlock) void k_check_csr(const int32_t* __restrict_
     if (lane_id() == 0 && bad) atomicAdd(&ctl->bad_csr, bad);
 }
 
+// ---------------------------------------------------- SCC size histogram
+// Extends get_scc_result (src/load_print.cpp:24-60), which tallies SCC
+// sizes in a std::map but prints only count classes.
+//
+// size[lab[v]] += 1 over canonical labels.  A wave takes 64 consecutive
+// vertices; lanes that share a label add together (one leader per round,
+// up to kSizeRounds rounds), the rest one atomic each, so a label carried by
+// a third of the vertices (R-MAT's giant) costs one atomic per wave step
+// instead of one per vertex.
+constexpr int kSizeRounds = 4;
+__global__ __launch_bounds__(kBlock) void k_scc_sizes(const int32_t* __restrict__ lab, int32_t V, int32_t* size) {
+    const int lane = lane_id();
+    const int64_t waves = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t base = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * kWave; base < V;
+         base += waves * kWave) {
+        const int64_t v = base + lane;
+        const int32_t l = v < V ? lab[v] : -1;
+        bool todo = v < V;
+        for (int r = 0; r < kSizeRounds; ++r) {
+            const uint64_t m = __ballot(todo);
+            if (!m) break;
+            const int lead = __ffsll((unsigned long long)m) - 1;
+            const int32_t L = lane_get(l, lead);
+            const uint64_t same = __ballot(todo && l == L);
+            if (lane == lead) atomicAdd(&size[L], __popcll(same));
+            if (l == L) todo = false;
+        }
+        if (todo) atomicAdd(&size[l], 1);
+    }
+}
+
+// Log2 bins of the SCC sizes (bin k: sizes in [2^k, 2^(k+1))), per block in
+// LDS, plus the largest size, the counts of sizes 1, 2, 3 and the SCC
+// count: out[0..63] bins, out[64] largest, out[65..67] sizes 1-3, out[68]
+// total.
+__global__ __launch_bounds__(kBlock) void k_scc_hist(const int32_t* __restrict__ size, int32_t V, int64_t* out) {
+    __shared__ int s_bin[64];
+    if (threadIdx.x < 64) s_bin[threadIdx.x] = 0;
+    __syncthreads();
+    int largest = 0, c1 = 0, c2 = 0, c3 = 0, tot = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const int c = size[v];
+        if (c <= 0) continue;
+        atomicAdd(&s_bin[31 - __clz(c)], 1);
+        largest = max(largest, c);
+        c1 += c == 1;
+        c2 += c == 2;
+        c3 += c == 3;
+        ++tot;
+    }
+    c1 = wave_sum(c1);
+    c2 = wave_sum(c2);
+    c3 = wave_sum(c3);
+    tot = wave_sum(tot);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) largest = max(largest, __shfl_xor(largest, d, kWave));
+    if (lane_id() == 0) {
+        if (largest) atomicMax(reinterpret_cast<unsigned long long*>(&out[64]), (unsigned long long)largest);
+        if (c1) atomicAdd(reinterpret_cast<unsigned long long*>(&out[65]), (unsigned long long)c1);
+        if (c2) atomicAdd(reinterpret_cast<unsigned long long*>(&out[66]), (unsigned long long)c2);
+        if (c3) atomicAdd(reinterpret_cast<unsigned long long*>(&out[67]), (unsigned long long)c3);
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(&out[68]), (unsigned long long)tot);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && s_bin[threadIdx.x])
+        atomicAdd(reinterpret_cast<unsigned long long*>(&out[threadIdx.x]), (unsigned long long)s_bin[threadIdx.x]);
+}
+
 // ------------------------------------------------------------- launchers
 
 #define LAUNCH(kern, grid, ...)                                  \
@@ -876,6 +944,14 @@ hipError_t launch_check_csr(const int32_t* beg, const int32_t* adj, int32_t V, i
 hipError_t launch_stats(const int32_t* fwb, const int32_t* bwb, const int32_t* lab, const uint64_t* fvis,
                         const uint64_t* bvis, int32_t V, Ctl* ctl, int grid, hipStream_t s) {
     LAUNCH(k_stats, grid, fwb, bwb, lab, fvis, bvis, V, ctl);
+}
+
+hipError_t launch_scc_histogram(const int32_t* lab, int32_t V, int32_t* size, int64_t* out, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid), dim3(kBlock), 0, s, size, 0, (int64_t)V);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, s, reinterpret_cast<int32_t*>(out), 0,
+                       (int64_t)kHistSlots * 2);
+    hipLaunchKernelGGL(k_scc_sizes, dim3(grid), dim3(kBlock), 0, s, lab, V, size);
+    LAUNCH(k_scc_hist, grid, size, V, out);
 }
 
 }  // namespace ispan

@@ -115,6 +115,7 @@ struct ispan_ctx {
     };
     std::vector<Rec> recs;
     ispan_kernel_profile prof{};
+    int64_t* hist = nullptr;  // kHistSlots results of ispan_scc_histogram
 };
 
 namespace {
@@ -912,6 +913,7 @@ int ispan_ctx_destroy(ispan_ctx* ctx) {
     free_workspace(ctx);
     dfree(ctx->ctl);
     dfree(ctx->gstate);
+    dfree(ctx->hist);
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
     if (ctx->h_gstate) (void)hipHostFree(ctx->h_gstate);
     for (auto& e : ctx->ev)
@@ -1642,6 +1644,33 @@ int ispan_summarize(int32_t V, const int32_t* labels, ispan_summary* out) {
         else ++out->others;
     }
     out->largest = largest;
+    return ISPAN_OK;
+}
+
+int ispan_scc_histogram(ispan_ctx* ctx, int64_t* bins, int32_t n_bins, ispan_summary* summary) {
+    if (!ctx || n_bins < 0 || n_bins > 64 || (n_bins > 0 && !bins)) return ISPAN_ERR_ARG;
+    if (!ctx->have_solution) return ISPAN_ERR_NOGRAPH;
+    int64_t h[kHistSlots] = {0};
+    if (ctx->V > 0) {
+        CK(hipSetDevice(ctx->device));
+        if (!ctx->hist) CK(hipMalloc(&ctx->hist, sizeof(h)));
+        // the mice queue buffer (int32[V + 1]) holds the sizes: the solve is over
+        CK(launch_scc_histogram(ctx->lab, ctx->V, ctx->qbuf, ctx->hist, ctx->grid, ctx->stream));
+        CK(hipMemcpyAsync(h, ctx->hist, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+    }
+    for (int k = 0; k < n_bins; ++k) bins[k] = h[k];
+    for (int k = n_bins; k < 64 && n_bins > 0; ++k) bins[n_bins - 1] += h[k];  // the last bin takes every larger size
+    if (summary) {
+        // the classes of ispan_summarize: the largest SCC is counted apart
+        // when it has more than 3 vertices
+        summary->largest = h[64];
+        summary->size_1 = h[65];
+        summary->size_2 = h[66];
+        summary->size_3 = h[67];
+        summary->total = h[68];
+        summary->others = h[68] - h[65] - h[66] - h[67] - (h[64] > 3 ? 1 : 0);
+    }
     return ISPAN_OK;
 }
 

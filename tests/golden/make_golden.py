@@ -12,8 +12,10 @@
                    the canonical labels (Tarjan), SCC count, giant size, and
                    whether the compiled reference agreed.
 
-Usage: python tests/golden/make_golden.py [--big]   (--big adds R-MAT s24 and
-the 4096^2 grid, which take a few minutes and ~10 GB of host memory.)
+Usage: python tests/golden/make_golden.py [--big] [--case KIND,SIZE,PARAM,SEED]
+(--big adds R-MAT s24 and the 4096^2 grid, which take a few minutes and
+~10 GB of host memory; --case hashes that one graph only, e.g.
+--case rmat,26,16,1 for R-MAT s26, ~30 GB of host memory.)
 """
 from __future__ import annotations
 
@@ -117,8 +119,12 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true")
     ap.add_argument("--toy-only", action="store_true")
+    ap.add_argument("--case", help="KIND,SIZE,PARAM,SEED: hash only this graph")
     a = ap.parse_args(argv)
     O.build()
+    if a.case:
+        kind, size, param, seed = a.case.split(",")
+        return hash_cases([(kind, int(size), int(param) if kind == "rmat" else float(param), int(seed))])
 
     # ---- toy fixture (reference data)
     fb = np.fromfile(os.path.join(REF_TOY, "fw_begin.bin"), np.int32)
@@ -161,15 +167,21 @@ def main(argv=None) -> int:
              ("grid", 512, 0.6, 1), ("grid", 1024, 0.6, 1)]
     if a.big:
         cases += [("rmat", 24, 16, 1), ("grid", 4096, 0.6, 1)]
+    return hash_cases(cases)
+
+
+def hash_cases(cases) -> int:
     path = os.path.join(HERE, "hashes.json")
     hashes = json.load(open(path)) if os.path.exists(path) else {}
     for kind, size, param, seed in cases:
         key = f"{kind}-{size}-{param}-seed{seed}"
         g = O.rmat(size, param, seed) if kind == "rmat" else O.grid(size, param, seed)
+        print(key, "generated", g.V, g.E, flush=True)
         tj = O.tarjan(g)
+        print(key, "tarjan done", flush=True)
         n_scc = int(np.count_nonzero(tj == np.arange(g.V)))
         giant = int(np.bincount(tj).max())
-        rl, why = ref_result(g, timeout=600.0)
+        rl, why = ref_result(g, timeout=1800.0)
         hashes[key] = {"kind": kind, "size": size, "param": param, "seed": seed, "V": g.V, "E": g.E,
                        "csr_sha256": csr_sha(g), "labels_sha256": sha(tj), "n_scc": n_scc, "giant": giant,
                        "ref": why if rl is None else ("agrees" if np.array_equal(rl, tj) else "wrong partition")}

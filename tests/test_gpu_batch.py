@@ -140,3 +140,37 @@ def test_batch_rejects_bad_graphs(solver):
     solver.upload(host(good))  # the context still solves single graphs
     solver.solve()
     assert np.array_equal(solver.labels(), O.tarjan(good))
+
+
+@pytest.mark.parametrize("max_group_edges", [0, 60_000_000])
+def test_batch_c4_distribution(solver, max_group_edges):
+    """256 graphs drawn as the batched workload draws them: V log-uniform in
+    [1e3, 1e6], edge factor uniform in [4, 16], (a,b,c,d) = (0.57,0.19,
+    0.19,0.05), generated on the device; one group, then split into groups of
+    at most 60M edges.  Every graph's labels equal Tarjan's on that graph."""
+    rng = np.random.default_rng(2024)
+    n = 256
+    Vs = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n)).astype(np.int64)
+    efs = rng.uniform(4.0, 16.0, n)
+    dgs = []
+    try:
+        for i in range(n):
+            dgs.append(solver.gen_rmat(int(Vs[i]), float(efs[i]), 77_000_003 + i))
+        b = I.DeviceBatch.from_device(dgs)
+        try:
+            solver.batch_bind(b, max_group_edges)
+        finally:
+            b.free()
+        if max_group_edges:
+            assert solver.batch_groups() >= 3
+        st = solver.batch_solve(I.default_params())
+        labs = solver.batch_labels()
+        giants = 0
+        for dg, lab in zip(dgs, labs):
+            hg = dg.download()
+            tj = O.tarjan(O.Graph(hg.fw_beg, hg.fw_adj, hg.bw_beg, hg.bw_adj))
+            assert np.array_equal(lab, tj), f"V={dg.V} E={dg.E}"
+        assert st.engine_used == 0 and st.giant_size > 0
+    finally:
+        for dg in dgs:
+            dg.free()

@@ -414,3 +414,100 @@ def test_autotune_picks_a_candidate_and_labels_do_not_change(solver):
     for p in cands:
         solver.solve(p)
         assert np.array_equal(solver.labels(), ref)
+
+
+# ------------------------------------------------------ SCC size histogram
+
+def size_bins(lab: np.ndarray, n_bins: int = 64) -> np.ndarray:
+    """Log2 bins of the SCC sizes of canonical labels (numpy, from the labels)."""
+    sizes = np.bincount(lab)
+    sizes = sizes[sizes > 0]
+    k = np.floor(np.log2(sizes)).astype(np.int64)
+    return np.bincount(np.minimum(k, n_bins - 1), minlength=n_bins)
+
+
+@pytest.mark.parametrize("name", ["toy", "rmat-16-16-seed1", "grid-512-0.6-seed1", "cycles", "pair_sink"])
+def test_scc_histogram(solver, name):
+    """Device histogram == bins of Tarjan's SCC sizes; its summary ==
+    ispan_summarize of the same labels."""
+    if name == "toy":
+        g = G.toy_graph()[0]
+    elif name == "cycles":
+        g = G.cycles_graph(3, n_cycles=300)
+    elif name == "pair_sink":
+        g = G.pattern_graph(dict(G.pattern_cases())[name])
+    else:
+        g = G.build_hashed(dict(G.hash_cases())[name])
+    lab, _ = solve(solver, g)
+    tj = O.tarjan(g)
+    assert np.array_equal(lab, tj)
+    bins, summ = solver.scc_histogram(64)
+    assert np.array_equal(bins, size_bins(tj))
+    assert summ == I.get_scc_result(tj)
+    assert int(bins.sum()) == summ["total"] == int(np.count_nonzero(tj == np.arange(g.V)))
+    b8, _ = solver.scc_histogram(8)  # the last bin takes every larger size
+    assert np.array_equal(b8, size_bins(tj, 8))
+
+
+def test_cli_histogram(tmp_path):
+    """`--histogram` adds one line per non-empty size range after each
+    result block; the result block itself is unchanged."""
+    d = dict(G.hash_cases())["rmat-16-16-seed1"]
+    g = G.build_hashed(d)
+    paths = O.write_ispan_files(g, str(tmp_path))
+    r = subprocess.run([I.CLI_PATH, *paths, "8", "30", "200", "10", "0.1", "2", "--histogram"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("\nSCC size histogram:\n") == 2
+    assert f"largest, {d['giant']}" in r.stdout and f"total, {d['n_scc']}" in r.stdout
+    block = r.stdout.split("\nSCC size histogram:\n")[1].split("\n\n")[0]
+    got = {}
+    for line in block.strip().splitlines():
+        key, n = line.split(", ")
+        lo = int(key.split()[1].split("-")[0])
+        got[int(np.log2(lo))] = int(n)
+    want = size_bins(O.tarjan(g))
+    assert got == {k: int(n) for k, n in enumerate(want) if n}
+    bad = subprocess.run([I.CLI_PATH, *paths, "8", "30", "200", "10", "0.1", "2", "--bogus"], capture_output=True,
+                         text=True, timeout=60)
+    assert bad.returncode == 255 and "Usage" in bad.stdout
+
+
+# ------------------------------------- switch rule and pivot product width
+
+def closed_form_pivot(g, wrap32: bool) -> int:
+    """Lowest id maximising out*in over vertices with both degrees > 0
+    (src/scc_common.h:92-150); wrap32 takes the product modulo 2^32 as a
+    signed int32, as the reference's index_t product does (:117)."""
+    od, idg = np.diff(g.fw_beg).astype(np.int64), np.diff(g.bw_beg).astype(np.int64)
+    p = od * idg
+    if wrap32:
+        p = ((p + 2 ** 31) % 2 ** 32) - 2 ** 31
+    p = np.where((od > 0) & (idg > 0), p, 0)
+    return int(np.argmax(p)) if p.max() > 0 else -1
+
+
+@pytest.mark.parametrize("wrap32", [0, 1])
+@pytest.mark.parametrize("engine", [0, 1])
+def test_pivot_wrap32_rmat21(solver, wrap32, engine):
+    """H3 at R-MAT s21: vertex 0's out*in passes 2^31, so the int32 product
+    picks another pivot.  The device pivot equals the closed form (and the
+    oracle restatement's) either way; the labels equal Tarjan's either way."""
+    g = O.rmat(21, 16, 1)
+    lab, st = solve(solver, g, pivot_wrap32=wrap32, engine=engine)
+    want = closed_form_pivot(g, bool(wrap32))
+    assert (want == 0) == (not wrap32)
+    assert st.pivot == want
+    assert O.ispan(g, threads=8, pivot_wrap32=bool(wrap32))[2]["pivot"] == want
+    assert np.array_equal(lab, O.tarjan(g))
+
+
+@pytest.mark.parametrize("key", ["rmat-20-16-seed1", "grid-1024-0.6-seed1", "rmat-18-8-seed3"])
+def test_reference_switch_estimate(solver, key):
+    """switch_rule = 0 (the reference's frontier estimate, src/fw_bw.h:578-581)
+    on the cooperative engine: same labels as the golden hash."""
+    d = dict(G.hash_cases())[key]
+    g = G.build_hashed(d)
+    lab, st = solve(solver, g, switch_rule=0)
+    assert G.sha(lab) == d["labels_sha256"]
+    assert st.engine_used == 0 and st.giant_size == d["giant"]

@@ -201,3 +201,45 @@ def test_submit_many_shares_one_batch(pool):
             srv.submit_many([ids[0], 999])
         assert e.value.code == -11
         assert srv.stats().requests_done == len(rids)
+
+
+def test_served_scales_interleaved(solver):
+    """R-MAT scales 12..16 (EF16) resident in one server; two client threads
+    submit interleaved requests (single submits and submit_many bursts);
+    every result equals Tarjan's labels of its graph."""
+    dgs = [solver.gen_rmat(1 << s, 16.0, 500 + 10 * s + k) for s in range(12, 17) for k in range(2)]
+    refs = []
+    try:
+        for dg in dgs:
+            hg = dg.download()
+            refs.append(O.tarjan(O.Graph(hg.fw_beg, hg.fw_adj, hg.bw_beg, hg.bw_adj)))
+        with I.Server(0) as srv:
+            ids = [srv.add_graph(dg) for dg in dgs]
+            errors, checked = [], []
+
+            def client(seed):
+                try:
+                    rng = np.random.default_rng(seed)
+                    for _ in range(6):
+                        picks = rng.integers(0, len(ids), int(rng.integers(1, 6)))
+                        if rng.random() < 0.5:
+                            rids = srv.submit_many([ids[int(k)] for k in picks])
+                        else:
+                            rids = [srv.submit(ids[int(k)]) for k in picks]
+                        for k, rid in zip(picks, rids):
+                            lab, info = srv.wait(rid)
+                            assert info.status == 0
+                            checked.append(bool(np.array_equal(lab, refs[int(k)])))
+                except Exception as e:  # noqa: BLE001 - reported below
+                    errors.append(e)
+
+            ts = [threading.Thread(target=client, args=(s,)) for s in (1, 2)]
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+            assert not errors, errors
+            assert checked and all(checked)
+    finally:
+        for dg in dgs:
+            dg.free()
