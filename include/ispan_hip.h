@@ -224,7 +224,8 @@ int ispan_autotune(ispan_ctx* ctx, const ispan_params* candidates, int32_t n, in
  * counterpart in the reference.  Registered graphs are appended once to an
  * arena (one CSR pair in device memory holding every resident graph side by
  * side); registration is checked and admitted against an HBM budget that
- * covers the arena, the solve state sized for it and the label buffers.  A
+ * covers the arena, the solve state sized for it and the label buffers, and
+ * against the device memory actually free when the arena has to grow.  A
  * request names a graph.  A worker thread takes every pending request within
  * the batch limits, solves the distinct graphs among them in one pass over
  * the arena (nothing is copied per batch) and starts over with what arrived
@@ -250,7 +251,9 @@ typedef struct {
 typedef struct {
     int64_t requests_done, batches, graphs_resident;
     int64_t bytes_resident;             /* the live graphs' share of the arena     */
-    int64_t bytes_budget, bytes_device; /* budget; arena + solve state + labels    */
+    int64_t bytes_budget, bytes_device; /* budget in effect (the configured one,
+                                           capped by device use + free memory);
+                                           arena + solve state + labels             */
     double p50_ms, p99_ms, mean_ms, max_ms; /* latency since the last reset        */
     double mean_batch_graphs;
     int64_t reserved[5];

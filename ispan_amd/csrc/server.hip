@@ -7,12 +7,18 @@
 //   memory holding every resident graph side by side with global ids, with
 //   its first-neighbour arrays.  Registration is admitted against an HBM
 //   budget that covers the arena, the solve state sized for it and the
-//   outstanding label buffers.
+//   outstanding label buffers, and against the device's free memory at the
+//   time (hipMemGetInfo) when the arena has to grow.
 // * A request names a graph.  The worker thread takes every pending request
 //   within the batch limits, solves the distinct graphs among them in one pass
 //   over the arena (a selection bitmap keeps every other vertex inactive, so
 //   nothing is copied per batch), and starts again with whatever arrived
-//   meanwhile.  Requests that arrive while a batch runs join the next one.
+//   meanwhile.  Requests that arrive while a batch runs join the next one:
+//   a batch is one cooperative launch whose blocks fill every CU and pass
+//   grid barriers level by level, so no second launch can run beside it, and
+//   a graph admitted into the running launch could only start at the next
+//   phase boundary (both traversals are seeded before their first level),
+//   which is no earlier than the next batch in practice.
 // * Labels stay in device memory (a pool of batch buffers) until the client
 //   takes them: copied to the host by wait(), or read in place through
 //   wait_device() and released.  Latency is submit -> labels ready on the
@@ -114,6 +120,9 @@ struct ispan_server {
 };
 
 namespace {
+
+// Device memory left free when the arena grows (runtime and other users).
+constexpr int64_t kFreeReserve = (int64_t)256 << 20;
 
 int64_t arena_bytes(int64_t V, int64_t E) { return 2 * (V + 1) * 4 + 2 * std::max<int64_t>(E, 1) * 4 + 2 * V * 4; }
 
@@ -344,9 +353,18 @@ int grow_arena(ispan_server* S, std::unique_lock<std::mutex>& lk, int64_t needV,
     int64_t nV = std::max<int64_t>(needV, std::min<int64_t>(2 * S->capV, INT32_MAX - 64));
     int64_t nE = std::max<int64_t>(needE, std::min<int64_t>(2 * S->capE, INT32_MAX));
     // doubling only while the budget allows it
-    // room is kept for the labels of one batch as large as the arena
+    // room is kept for the labels of one batch as large as the arena.  Besides
+    // the configured budget, the growth must fit the device's free memory
+    // now: the new arena is allocated while the old one is still alive, and
+    // the solve state and the label room grow to the new size.
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return ISPAN_ERR_HIP;
+    const int64_t cur = arena_bytes(S->capV, S->capE) + ispan::solve_state_bytes(S->capV, S->capE) +
+                        4 * (S->capV + 1);
     auto fits = [&](int64_t v, int64_t e) {
-        return arena_bytes(v, e) + ispan::solve_state_bytes(v, e) + 4 * (v + 1) <= S->budget;
+        const int64_t need = arena_bytes(v, e) + ispan::solve_state_bytes(v, e) + 4 * (v + 1);
+        const int64_t extra = need - cur + arena_bytes(S->capV, S->capE);
+        return need <= S->budget && extra + kFreeReserve <= (int64_t)free_b;
     };
     if (!fits(nV, nE)) {
         nV = needV;
@@ -661,8 +679,13 @@ int ispan_server_stats_get(ispan_server* S, ispan_server_stats* out) {
     for (const auto& kv : S->graphs) live += !kv.second.removed;
     out->graphs_resident = live;
     out->bytes_resident = S->bytes_live;
-    out->bytes_budget = S->budget;
     out->bytes_device = bytes_used(S);
+    // the budget in effect: the configured one, capped by what the device
+    // could still give this server
+    size_t free_b = 0, total_b = 0;
+    out->bytes_budget = S->budget;
+    if (hipSetDevice(S->device) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        out->bytes_budget = std::min<int64_t>(S->budget, out->bytes_device + (int64_t)free_b);
     out->p50_ms = percentile(S->latencies, 0.50);
     out->p99_ms = percentile(S->latencies, 0.99);
     double sum = 0, mx = 0;

@@ -24,7 +24,8 @@ def rt():
                         ("hipMemcpy", [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
                         ("hipStreamCreate", [C.POINTER(C.c_void_p)]), ("hipStreamDestroy", [C.c_void_p]),
                         ("hipStreamSynchronize", [C.c_void_p]), ("hipDeviceSynchronize", []),
-                        ("hipDeviceReset", [])):
+                        ("hipDeviceReset", []),
+                        ("hipMemGetInfo", [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)])):
             getattr(r, n).argtypes = args
             getattr(r, n).restype = C.c_int
         _rt = r
@@ -96,6 +97,13 @@ def malloc(nbytes: int) -> int:
 
 def free(ptr: int):
     _ok(rt().hipFree(ptr), "hipFree")
+
+
+def mem_info() -> tuple[int, int]:
+    """(free, total) device memory in bytes (hipMemGetInfo)."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _ok(rt().hipMemGetInfo(C.byref(f), C.byref(t)), "hipMemGetInfo")
+    return f.value, t.value
 
 
 def synchronize():

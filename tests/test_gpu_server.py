@@ -243,3 +243,38 @@ def test_served_scales_interleaved(solver):
     finally:
         for dg in dgs:
             dg.free()
+
+
+def test_free_memory_caps_admission(solver):
+    """A configured budget far above the device's memory admits only what the
+    device can still hold: with all but ~1.5 GB of HBM taken by another
+    allocation, a graph whose arena and solve state need ~2 GB is rejected
+    (ISPAN_ERR_BUDGET) although the 16 TiB budget would take it; once that
+    memory is released, the same graph is admitted and solved."""
+    from ispan_amd import hiprt
+    dg = solver.gen_rmat(1 << 23, 16.0, 9)
+    hog = []
+    try:
+        _, total = hiprt.mem_info()
+        with I.Server(0, hbm_budget=1 << 44) as srv:
+            assert srv.stats().bytes_budget <= total
+            while True:  # take memory in pieces until ~1.5 GB is left
+                free, _ = hiprt.mem_info()
+                if free <= (1536 << 20):
+                    break
+                hog.append(hiprt.malloc(min(free - (1536 << 20), 8 << 30)))
+            with pytest.raises(I.IspanError) as e:
+                srv.add_graph(dg)
+            assert e.value.code == -9
+            while hog:
+                hiprt.free(hog.pop())
+            gid = srv.add_graph(dg)
+            lab, info = srv.wait(srv.submit(gid))
+            assert info.status == 0
+        solver.bind(dg)
+        solver.solve(I.default_params())
+        assert np.array_equal(lab, solver.labels())
+    finally:
+        while hog:
+            hiprt.free(hog.pop())
+        dg.free()
