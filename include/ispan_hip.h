@@ -59,7 +59,16 @@ typedef struct ispan_params {
     int32_t pivot_wrap32; /* 1: reproduce the reference's int32 out*in product      */
     int32_t timing;       /* 1: per-phase HIP event timing into ispan_stats.ms      */
     int32_t validate;     /* 1: check CSR structure when a graph is bound           */
-    int32_t wcc_mode;     /* 0: union-find hooking, 1: label propagation + jumping  */
+    int32_t wcc_mode;     /* small SCCs after the trims: 0 WCC by union-find
+                             hooking (openmp_wcc's union, src/openmp_wcc.hpp),
+                             1 WCC by label propagation + jumping (coloring_wcc,
+                             src/color_propagation.h:468-593), each followed by
+                             FW-BW per WCC (mice); 2 the colouring SCC method
+                             (graph_color, src/color_propagation.h:9-467), no
+                             mice phase.  Engine 0 runs all three on the device
+                             with no host round trip; engine 1 checks label
+                             propagation from the host each round and runs 2
+                             as 0.  Chooses the work, never the labels.       */
     int32_t async_batch;  /* engine 1: top-down levels enqueued per host check (8)  */
     int32_t engine;       /* giant phase: 0 one cooperative persistent kernel with
                              grid barriers (default), 1 one launch per level with
@@ -69,7 +78,12 @@ typedef struct ispan_params {
                              work is its exact out-degree sum; 0 it is found *
                              avg_deg, as src/fw_bw.h:578-581.  Engine 1 always
                              uses 0.  Chooses the work, never the labels.       */
-    int32_t reserved[3];
+    int32_t trim_scope;   /* size-2/3 pattern trims walk: 0 the active queue
+                             (trim_*_from_fq, src/trim_2_3.h:146-1080), 1 every
+                             active vertex of the graph (trim_*_from_graph,
+                             src/trim_2_3.h:9-144, 287-480, 682-879); engine 0
+                             only.  Same detections, different work.          */
+    int32_t reserved[2];
 } ispan_params;
 
 /* Per-solve statistics.  ms[] uses the reference's avg_time slot map

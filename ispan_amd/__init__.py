@@ -62,7 +62,7 @@ class Params(C.Structure):
                 ("trim_times", C.c_int32), ("thread_count", C.c_int32), ("pivot_wrap32", C.c_int32),
                 ("timing", C.c_int32), ("validate", C.c_int32), ("wcc_mode", C.c_int32),
                 ("async_batch", C.c_int32), ("engine", C.c_int32), ("switch_rule", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("trim_scope", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):

@@ -83,7 +83,9 @@ struct GiantState {
     int64_t ts[6];      // wall-clock stamps: start, pivot known, FW done, BW done, giant relabelled, end
     GLine fqn[3];       // active-queue lengths of the in-kernel trim sweeps (rotating)
     int64_t n_final, final_sel, trim1, pat2, pat3, sweeps, wcc_done;
-    int64_t pad[5];
+    int64_t tail_rounds;  // label-propagation sweeps (wcc_mode 1) or colouring rounds (2)
+    int64_t tail_sccs;    // colouring: SCCs it labelled
+    int64_t pad[3];
     RLine gacc[32];     // barrier reductions: per arrival group
     RLine tacc;         // barrier reductions: over groups
     RLine res[2];       // barrier reductions: result of generation g in res[g & 1]
@@ -155,6 +157,11 @@ struct GiantArgs {
     int64_t bu_mult;          // coprime to grid * waves per block: bottom-up run permutation
     int32_t switch_rule;      // top-down -> bottom-up estimate of the frontier's work: 0 found x average
                               // degree (the reference's), 1 the frontier's exact out-degree sum
+    int32_t wcc_mode;         // after the trims: 0 union-find WCC, 1 label-propagation WCC, 2 the
+                              // colouring SCC method (no mice phase left)
+    int32_t trim_scope;       // size-2/3 pattern walk: 0 the active queue, 1 every active vertex
+    uint64_t* ckey;           // colouring: (priority, colour) per vertex
+    int32_t* aux;             // colouring: smallest member per SCC root
 };
 
 constexpr int kGiantSlots = 16;
@@ -170,6 +177,10 @@ int64_t giant_bu_mult(int grid);
 // zeroes a.gs and ctl, records `before` (when given), then launches k_giant
 // cooperatively
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before);
+// after launch_giant on the same stream, for a.wcc_mode 1 or 2 when the giant
+// phase left active vertices: the WCC by label propagation, or the colouring
+// SCC method, over them
+hipError_t launch_tail(const GiantArgs& a, int grid, hipStream_t s);
 // batched solves (batch.hip)
 hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, const int32_t* bwb_cat,
                               const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,

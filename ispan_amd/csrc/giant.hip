@@ -39,7 +39,8 @@ struct Smem {
     int64_t rres[4];  // reduction result of the last barrier
     int32_t bi[kW];
     int ok;
-    int32_t nbar;  // barriers this block has arrived at
+    int32_t nbar;   // barriers this block has arrived at in this launch
+    int32_t gbase;  // barrier generation when this launch started
     int32_t wbuf[kW][kWaveBuf];
     int32_t wcnt[2][kW];    // block flushes: per-wave counts and bases of up to two counters
     int32_t wbase[2][kW];
@@ -107,10 +108,11 @@ __device__ __forceinline__ bool gsync(GiantState* gs, int G, Smem& sm, int32_t* 
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        // gen starts at 0 in every launch and each barrier bumps it once, so
-        // at this block's nb-th barrier it reads nb - 1: no load needed
+        // gen is gbase when the launch starts and each barrier bumps it once,
+        // so at this block's nb-th barrier it reads gbase + nb - 1: no load
+        // needed
         const int32_t nb = ++sm.nbar;
-        const int32_t g0 = nb - 1;
+        const int32_t g0 = sm.gbase + nb - 1;
         const int grp = blockIdx.x % kBarGroups;
         const int ngrp = min(kBarGroups, G);
         const int gsize = (G - grp + kBarGroups - 1) / kBarGroups;
@@ -957,6 +959,199 @@ __device__ __forceinline__ int32_t uf_root_g(int32_t* parent, int32_t v) {
     return v;
 }
 
+// ---- small-SCC strategies after the trims (wcc_mode 1 and 2), run by
+// k_tail: a second cooperative launch on the same stream, made only when the
+// giant phase leaves active vertices.  A kernel of their own keeps k_giant's
+// code (and its register allocation) as it is.
+// col[v] = smallest id of v's WCC in the active subgraph, by label
+// propagation over in- and out-edges with jumps along the colour chain, to a
+// fixed point (coloring_wcc, src/color_propagation.h:468-593, which keeps
+// the largest id).  The count of changed vertices rides on the barrier, so
+// every block leaves after the same sweep.
+__device__ __forceinline__ bool lp_wcc(Ctx& c, const int32_t* q, int64_t n, int64_t& rounds) {
+    const GiantArgs& a = c.a;
+    const int64_t nthreads = (int64_t)c.G * kGBlock;
+    const int64_t gt = (int64_t)c.b * kGBlock + c.tid;
+    int32_t* col = a.parent;
+    for (int64_t i = gt; i < n; i += nthreads) st_agent(&col[q[i]], q[i]);
+    if (!gsync(a.gs, c.G, c.sm, a.dbg)) return false;
+    for (rounds = 1;; ++rounds) {
+        int changed = 0;
+        for (int64_t i = gt; i < n; i += nthreads) {
+            const int32_t v = q[i];
+            const int32_t c0 = load_agent(&col[v]);
+            int32_t m = c0;
+            for (int dir = 0; dir < 2; ++dir) {
+                const int32_t* beg = dir ? a.bwb : a.fwb;
+                const int32_t* adj = dir ? a.bwa : a.fwa;
+                for (int32_t k = beg[v]; k < beg[v + 1]; ++k) {
+                    const int32_t w = adj[k];
+                    if (w != v && bit_of(a.act, w)) m = min(m, load_agent(&col[w]));
+                }
+            }
+            // colours are ids of active members: follow the chain (at most
+            // 100 hops, as the reference's path compression)
+            for (int hop = 0; hop < 100; ++hop) {
+                const int32_t nx = load_agent(&col[m]);
+                if (nx >= m) break;
+                m = nx;
+            }
+            if (m < c0) {
+                atomicMin(&col[v], m);
+                ++changed;
+            }
+        }
+        Red r;
+        if (!gsync<true>(a.gs, c.G, c.sm, a.dbg, Red{block_sum(changed, c.sm), 0, 0, 0}, &r, 1)) return false;
+        if (r.s0 == 0) return true;
+        if (rounds > (int64_t)a.V + 2) {
+            if (c.tid == 0) atomicOr(&a.gs->err.v, 2);
+            return false;
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t key_colour(uint64_t k) { return (int32_t)(uint32_t)k; }
+
+// The colouring SCC method (graph_color, src/color_propagation.h:9-467).  A
+// vertex's priority is (active out-degree + 5) x active in-degree
+// (degree_rank, :9-58; clipped to 31 bits here, which can only change ties:
+// the priority chooses the work, never the labels).  Each round every active
+// vertex takes the largest (priority, id) key among the vertices that reach
+// it: propagation over in-edges with jumps along the colour chain
+// (color_propagation, :64-200), to a fixed point.  A vertex whose key is its
+// own is a root, and the vertices of its colour that reach it are its SCC
+// (color_identify, :203-260; here a fixed point of backward sweeps for all
+// roots at once instead of one search per root).  Those SCCs are labelled
+// with their smallest id and leave the active set; the rest start the next
+// round with their own keys again (color_init, :262-290).  Each round labels
+// at least the SCC of the largest key of every weakly connected piece.
+__device__ __forceinline__ bool colour_scc(Ctx& c, int& cur, int64_t n, int s, int64_t& rounds, int64_t& sccs) {
+    const GiantArgs& a = c.a;
+    GiantState* gs = a.gs;
+    const int64_t nthreads = (int64_t)c.G * kGBlock;
+    const int64_t gt = (int64_t)c.b * kGBlock + c.tid;
+    auto buf = [&a](int k) { return k ? a.fq : a.fq_alt; };
+    using ull = unsigned long long;
+    {  // degree rank, once, into parent
+        const int32_t* q = buf(cur);
+        for (int64_t i = gt; i < n; i += nthreads) {
+            const int32_t v = q[i];
+            int64_t od = 0, id = 0;
+            for (int32_t k = a.fwb[v]; k < a.fwb[v + 1]; ++k) od += bit_of(a.act, a.fwa[k]);
+            for (int32_t k = a.bwb[v]; k < a.bwb[v + 1]; ++k) id += bit_of(a.act, a.bwa[k]);
+            st_agent(&a.parent[v], (int32_t)min<int64_t>((od + 5) * id, INT32_MAX));
+        }
+    }
+    sccs = 0;
+    for (rounds = 1;; ++rounds) {
+        const int32_t* q = buf(cur);
+        int32_t* qn = buf(cur ^ 1);
+        int32_t* cnt = &gs->fqn[s % 3].v;  // last read three rounds (or sweeps) ago
+        if (c.b == 0 && c.tid == 0) __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int64_t i = gt; i < n; i += nthreads) {
+            const int32_t v = q[i];
+            st_agent(&a.ckey[v], ((uint64_t)(uint32_t)load_agent(&a.parent[v]) << 32) | (uint32_t)v);
+            st_agent(&a.stamp[v], -1);
+        }
+        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
+        for (int64_t it = 0;; ++it) {  // largest key among the vertices that reach v
+            int changed = 0;
+            for (int64_t i = gt; i < n; i += nthreads) {
+                const int32_t v = q[i];
+                const uint64_t k0 = load_agent(&a.ckey[v]);
+                uint64_t k = k0;
+                for (int32_t e = a.bwb[v]; e < a.bwb[v + 1]; ++e) {
+                    const int32_t w = a.bwa[e];
+                    if (w != v && bit_of(a.act, w)) k = max(k, load_agent(&a.ckey[w]));
+                }
+                for (int hop = 0; hop < 100; ++hop) {  // the colour's own key, and so on
+                    const uint64_t kc = load_agent(&a.ckey[key_colour(k)]);
+                    if (kc <= k) break;
+                    k = kc;
+                }
+                if (k > k0) {
+                    atomicMax(reinterpret_cast<ull*>(&a.ckey[v]), (ull)k);
+                    ++changed;
+                }
+            }
+            Red r;
+            if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{block_sum(changed, c.sm), 0, 0, 0}, &r, 1)) return false;
+            if (r.s0 == 0) break;
+            if (it > (int64_t)a.V + 2) {
+                if (c.tid == 0) atomicOr(&gs->err.v, 2);
+                return false;
+            }
+        }
+        for (int64_t i = gt; i < n; i += nthreads) {  // roots
+            const int32_t v = q[i];
+            if (key_colour(load_agent(&a.ckey[v])) == v) {
+                st_agent(&a.stamp[v], v);
+                st_agent(&a.aux[v], v);
+            }
+        }
+        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
+        for (int64_t it = 0;; ++it) {  // v joins its colour's SCC through an out-neighbour in it
+            int changed = 0;
+            for (int64_t i = gt; i < n; i += nthreads) {
+                const int32_t v = q[i];
+                if (load_agent(&a.stamp[v]) >= 0) continue;
+                const int32_t r = key_colour(load_agent(&a.ckey[v]));
+                for (int32_t e = a.fwb[v]; e < a.fwb[v + 1]; ++e) {
+                    const int32_t u = a.fwa[e];
+                    if (u != v && bit_of(a.act, u) && load_agent(&a.stamp[u]) == r) {
+                        st_agent(&a.stamp[v], r);
+                        ++changed;
+                        break;
+                    }
+                }
+            }
+            Red r;
+            if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{block_sum(changed, c.sm), 0, 0, 0}, &r, 1)) return false;
+            if (r.s0 == 0) break;
+            if (it > (int64_t)a.V + 2) {
+                if (c.tid == 0) atomicOr(&gs->err.v, 2);
+                return false;
+            }
+        }
+        for (int64_t i = gt; i < n; i += nthreads) {  // smallest member per SCC
+            const int32_t v = q[i];
+            const int32_t r = load_agent(&a.stamp[v]);
+            if (r >= 0 && r != v) atomicMin(&a.aux[r], v);
+        }
+        if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
+        // label the SCCs found, take them out of the active set, queue the rest
+        WaveQueue wq{c.sm.wbuf[c.wave], 0};
+        int found = 0;
+        for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < n; ib += nthreads) {
+            const int64_t i = ib + c.lane;
+            int32_t v = 0, r = -1;
+            if (i < n) {
+                v = q[i];
+                r = load_agent(&a.stamp[v]);
+                if (r >= 0) {
+                    st_agent(&a.lab[v], load_agent(&a.aux[r]));
+                    atomicAnd(reinterpret_cast<ull*>(a.act + (v >> 6)), ~(1ull << (v & 63)));
+                    found += r == v;
+                }
+            }
+            wq_push<true>(wq, i < n && r < 0, v, qn, cnt);
+        }
+        block_flush(c, wq, qn, cnt);
+        Red r;
+        if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{block_sum(found, c.sm), 0, 0, 0}, &r, 1)) return false;
+        sccs += r.s0;
+        n = load_agent(cnt);
+        cur ^= 1;
+        ++s;
+        if (n == 0) return true;
+        if (rounds > (int64_t)a.V + 2) {
+            if (c.tid == 0) atomicOr(&gs->err.v, 2);
+            return false;
+        }
+    }
+}
+
 // Trims and WCC after the giant.  Returns false after a failed barrier.
 __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
     const GiantArgs& a = c.a;
@@ -992,22 +1187,38 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
     }
     bool wcc = false;
     if (n > 0) {
-        // P8-P10: pattern detection against the current act, then commit
+        // P8-P10: pattern detection against the current act, then commit.
+        // Over the active queue (trim_*_from_fq, src/trim_2_3.h:146-283,
+        // 482-677, 881-1080) or, with trim_scope 1, over every active vertex
+        // of the graph (trim_*_from_graph, src/trim_2_3.h:9-144, 287-480,
+        // 682-879): the same detections, a different walk.
         const int32_t* f = buf(cur);
-        for (int64_t i = gt; i < n; i += nthreads) {
-            const int32_t v = f[i];
-            if (bit_of(a.act, v)) pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v);
-        }
+        auto each_active = [&](auto&& fn) {
+            if (a.trim_scope) {
+                for (int64_t w = gt; w < a.nwords; w += nthreads) {
+                    uint64_t m = a.act[w];
+                    while (m) {
+                        fn((int32_t)(w * 64 + __ffsll((unsigned long long)m) - 1));
+                        m &= m - 1;
+                    }
+                }
+            } else {
+                for (int64_t i = gt; i < n; i += nthreads) {
+                    const int32_t v = f[i];
+                    if (bit_of(a.act, v)) fn(v);
+                }
+            }
+        };
+        each_active([&](int32_t v) { pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v); });
         if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         int c2 = 0, c3 = 0;
-        for (int64_t i = gt; i < n; i += nthreads) {
-            const int32_t v = f[i];
-            if (a.lab[v] >= 0 && bit_of(a.act, v)) {
+        each_active([&](int32_t v) {
+            if (a.lab[v] >= 0) {
                 atomicAnd(reinterpret_cast<unsigned long long*>(a.act + (v >> 6)), ~(1ull << (v & 63)));
                 if (a.stamp[v] == 2) ++c2;
                 else ++c3;
             }
-        }
+        });
         const int64_t b2 = block_sum(c2, c.sm), b3 = block_sum(c3, c.sm);
         Red r;
         if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{b2, b3, 0, 0}, &r, 3)) return false;
@@ -1015,7 +1226,8 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
         p3 = r.s1;
         // P11: one more trim-1 sweep
         if (!sweep()) return false;
-        if (n > 0) {
+        // (wcc_mode 1 and 2 run in k_tail, launched when something is left)
+        if (n > 0 && a.wcc_mode == 0) {
             // P12: WCC by union-find; parent = smallest id of the WCC
             const int32_t* g = buf(cur);
             for (int64_t i = gt; i < n; i += nthreads) {
@@ -1071,6 +1283,7 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         a.dbg[blockIdx.x] = 0;
         a.dbg[gridDim.x + blockIdx.x] = 0;
         sm.nbar = 0;
+        sm.gbase = 0;  // the state is cleared before every launch
     }
     Ctx c{a, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, lane_id(), wave_id(), sm, 0, 0, 0, 0};
     GiantState* gs = a.gs;
@@ -1327,6 +1540,43 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     }
 }
 
+__global__ __launch_bounds__(kGBlock) void k_tail(const GiantArgs a) {
+    __shared__ Smem sm;
+    GiantState* gs = a.gs;
+    if (threadIdx.x == 0) {
+        a.dbg[blockIdx.x] = 0;
+        a.dbg[gridDim.x + blockIdx.x] = 0;
+        sm.nbar = 0;
+        sm.gbase = gs->gen.v;  // barrier generations continue from k_giant's
+        sm.ok = gs->err.v == 0;
+    }
+    __syncthreads();
+    if (!sm.ok) return;  // k_giant failed; the host reports its error
+    const int64_t n = gs->n_final;
+    int cur = (int)gs->final_sel;
+    if (n == 0) return;
+    Ctx c{a, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, lane_id(), wave_id(), sm, 0, 0, 0, 0};
+    const bool lead = c.b == 0 && c.tid == 0;
+    int64_t rounds = 0, sccs = 0;
+    if (a.wcc_mode == 1) {  // P12 by label propagation: parent = smallest id of the WCC
+        if (!lp_wcc(c, cur ? a.fq : a.fq_alt, n, rounds)) return;
+        if (lead) {
+            gs->wcc_done = 1;
+            gs->tail_rounds = rounds;
+        }
+    } else if (a.wcc_mode == 2) {  // P12-P14 replaced: nothing is left for the mice phase
+        if (!colour_scc(c, cur, n, (int)gs->sweeps, rounds, sccs)) return;
+        if (lead) {
+            gs->n_final = 0;
+            gs->final_sel = cur;
+            gs->tail_rounds = rounds;
+            gs->tail_sccs = sccs;
+            a.fq_n_ctl[cur] = 0;
+        }
+    }
+    if (lead) gs->ts[5] = wall_clock64();
+}
+
 __global__ void k_zero_i64(int64_t* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -1377,6 +1627,12 @@ hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, h
     GiantArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args, 0, s);
+}
+
+hipError_t launch_tail(const GiantArgs& a, int grid, hipStream_t s) {
+    GiantArgs copy = a;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tail), dim3(grid), dim3(kGBlock), args, 0, s);
 }
 
 hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s) {

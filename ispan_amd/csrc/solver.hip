@@ -474,6 +474,10 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.dbg = ctx->gdbg;
         ga.trace = ctx->level_trace ? 1 : 0;
         ga.bu_mult = giant_bu_mult(ctx->coop_grid);
+        ga.wcc_mode = P->wcc_mode;
+        ga.trim_scope = P->trim_scope;
+        ga.ckey = ctx->keys;
+        ga.aux = ctx->qbuf;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
         if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) ctx->recs.push_back({3, ga0, ga1});
         const hipError_t le = launch_giant(ga, ctx->ctl, ctx->coop_grid, s, ga0);
@@ -492,6 +496,17 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             // ones now, so that one host wait covers the solve
             CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
             if ((rc = readback(ctx, &syncs))) return rc;
+            if (!ctx->h_gstate->err.v && ctx->h_gstate->n_final > 0 && (P->wcc_mode == 1 || P->wcc_mode == 2)) {
+                // active vertices are left: the WCC by label propagation, or
+                // the colouring SCC method, on the device (one more launch
+                // and wait, as the mice phase would take)
+                CK(launch_tail(ga, ctx->coop_grid, s));
+                CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate,
+                                  ctx->level_trace ? sizeof(GiantState) : kGiantStateHead, hipMemcpyDeviceToHost, s));
+                CK(launch_fill_i32(&ctx->ctl->unlabelled, 0, 1, 1, s));
+                CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
+                if ((rc = readback(ctx, &syncs))) return rc;
+            }
             const GiantState& gs = *ctx->h_gstate;
             if (gs.err.v) {
                 if (gs.err.v & 1) {  // where did the blocks stop?
@@ -684,8 +699,11 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     }
     const bool host_tail = n > 0;
     if (n > 0) {
-        // P12 WCC of the remaining subgraph
-        if (P->wcc_mode == 1) {
+        // P12 WCC of the remaining subgraph (engine 0 has done it on the
+        // device; engine 1 runs label propagation with a host check per
+        // round, and the union-find otherwise, also for wcc_mode 2)
+        if (wcc_ready) {
+        } else if (P->wcc_mode == 1) {
             CK(launch_uf_init(ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
             int it = 0;
             for (;; ++it) {
@@ -698,7 +716,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                 if ((rc = readback(ctx, &syncs))) return rc;
                 if (!ctx->h_ctl->changed) break;
             }
-        } else if (!wcc_ready) {
+        } else {
             CK(launch_wcc_uf(ctx->fwb, ctx->fwa, ctx->act, ctx->fq[sel], ctx->parent, ctx->ctl, sel, G, s));
         }
         rec(ctx, P, 8);

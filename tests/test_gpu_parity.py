@@ -42,10 +42,10 @@ def test_toy(solver):
 
 
 @pytest.mark.parametrize("name,d", G.pattern_cases(), ids=[n for n, _ in G.pattern_cases()])
-@pytest.mark.parametrize("wcc_mode", [0, 1])
-def test_pattern_graphs(solver, name, d, wcc_mode):
+@pytest.mark.parametrize("wcc_mode,trim_scope", [(0, 0), (1, 0), (2, 0), (0, 1), (2, 1)])
+def test_pattern_graphs(solver, name, d, wcc_mode, trim_scope):
     g = G.pattern_graph(d)
-    lab, st = solve(solver, g, wcc_mode=wcc_mode)
+    lab, st = solve(solver, g, wcc_mode=wcc_mode, trim_scope=trim_scope)
     assert lab.tolist() == d["canonical"]
     if name.startswith("pair"):
         assert st.trim2 == 2
@@ -57,7 +57,9 @@ def test_pattern_graphs(solver, name, d, wcc_mode):
 
 PARAMS = [dict(), dict(wcc_mode=1), dict(pivot_wrap32=1), dict(alpha=1, beta=1, gamma=0),
           dict(alpha=1000000, beta=2, gamma=1), dict(alpha=2, beta=1000000, gamma=100, async_batch=1),
-          dict(theta=0.9, trim_times=0), dict(theta=0.0, trim_times=5)]
+          dict(theta=0.9, trim_times=0), dict(theta=0.0, trim_times=5), dict(wcc_mode=2), dict(trim_scope=1),
+          dict(wcc_mode=2, trim_scope=1, theta=0.9, trim_times=0), dict(wcc_mode=1, engine=1), dict(switch_rule=0),
+          dict(wcc_mode=1, trim_times=0)]
 
 
 @pytest.mark.parametrize("seed", range(96))
@@ -392,7 +394,7 @@ def test_convert_order_duplicates_and_clean(solver, tmp_path):
 @pytest.mark.parametrize("seed", range(16))
 def test_engines_random(solver, engine, seed):
     g = G.random_graph(500 + seed, vmax=[60, 800, 5000][seed % 3], density=[0.7, 2.0, 5.0][seed % 3])
-    check(solver, g, engine=engine, **PARAMS[seed % len(PARAMS)])
+    check(solver, g, **dict(PARAMS[seed % len(PARAMS)], engine=engine))
 
 
 @pytest.mark.parametrize("engine", [0, 1])
@@ -511,3 +513,30 @@ def test_reference_switch_estimate(solver, key):
     lab, st = solve(solver, g, switch_rule=0)
     assert G.sha(lab) == d["labels_sha256"]
     assert st.engine_used == 0 and st.giant_size == d["giant"]
+
+
+
+# ----------------------------------------------- small-SCC strategy arms
+
+@pytest.mark.parametrize("key", ["rmat-20-16-seed1", "rmat-18-8-seed3", "grid-1024-0.6-seed1", "grid-4096-0.6-seed1"])
+@pytest.mark.parametrize("wcc_mode,trim_scope", [(1, 0), (2, 0), (2, 1), (0, 1)])
+def test_small_scc_arms_hashed(solver, key, wcc_mode, trim_scope):
+    """Every small-SCC strategy on the cooperative engine gives the golden
+    labels: label-propagation WCC then mice (1), the colouring SCC method (2),
+    and the full-graph pattern walk (trim_scope 1).  The colouring method
+    leaves nothing to the host-driven mice phase: at most one more launch
+    and host wait after the giant phase's."""
+    d = dict(G.hash_cases())[key]
+    dg = solver.gen_rmat(1 << d["size"], float(d["param"]), d["seed"]) if d["kind"] == "rmat" else \
+        solver.gen_grid(d["size"], float(d["param"]), d["seed"])
+    try:
+        solver.bind(dg)
+        st = solver.solve(I.default_params(wcc_mode=wcc_mode, trim_scope=trim_scope))
+        assert G.sha(solver.labels()) == d["labels_sha256"]
+        assert st.engine_used == 0
+        if wcc_mode == 2:
+            assert st.host_syncs <= 2 and st.mice == 0
+        elif st.mice > 0:
+            assert st.n_wcc > 0
+    finally:
+        dg.free()
