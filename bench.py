@@ -111,9 +111,10 @@ def batch_leg(S, I, n_graphs: int, seed: int, steps: int, warmup: int) -> dict:
             st = S.batch_solve(p)
         S.synchronize()
         batch_ms = (time.perf_counter() - t0) * 1e3 / steps
-        labs = S.batch_labels()
+        labs = [x.copy() for x in S.batch_labels()]
         groups = S.batch_groups()
         log(f"[batch] {groups} groups, {batch_ms:.3f} ms per batch")
+        tuned = class_tuned(S, I, dgs, labs, steps, warmup)
         # one graph per launch
         single_ms, equal = 0.0, True
         for dg, lab in zip(dgs, labs):
@@ -128,13 +129,54 @@ def batch_leg(S, I, n_graphs: int, seed: int, steps: int, warmup: int) -> dict:
                "edges_per_s": sum_E / (batch_ms * 1e-3),
                "one_per_launch": {"ms_total": single_ms, "graphs_per_s": n_graphs / (single_ms * 1e-3)},
                "speedup_vs_one_per_launch": single_ms / batch_ms, "labels_equal_one_per_launch": equal,
-               "giant_total": int(st.giant_size), "gen_s": gen_s, "bind_s": bind_s,
+               "giant_total": int(st.giant_size), "gen_s": gen_s, "bind_s": bind_s, "tuned_by_size_class": tuned,
                "workload": f"{n_graphs} R-MAT graphs, V log-uniform [1e3, 1e6], EF uniform [4, 16], "
                            f"(0.57,0.19,0.19,0.05), seeds {seed}*1000003+i, generated on the device"}
         log(f"[batch] one per launch {single_ms:.1f} ms, speedup {single_ms / batch_ms:.1f}x, labels equal {equal}")
     finally:
         for dg in dgs:
             dg.free()
+    return out
+
+
+def class_tuned(S, I, dgs, ref_labels, steps: int, warmup: int) -> dict:
+    """The same batch bound by size class (floor(log2 V)), timed with the
+    default parameters, then tuned per class (the project's autotuner over
+    class_tuning_grid) and timed with each class's choice.  Labels are
+    compared graph by graph with the default solve's."""
+    t0 = time.time()
+    b = I.DeviceBatch.from_device(dgs)
+    try:
+        S.batch_bind(b, 0, I.default_params(validate=0, batch_by_class=1))
+    finally:
+        b.free()
+    groups = S.batch_groups()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        S.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        S.synchronize()
+        return (time.perf_counter() - t1) * 1e3 / steps
+
+    by_class_ms = timed(lambda: S.batch_solve(I.default_params()))
+    t2 = time.time()
+    cands = I.class_tuning_grid()
+    table = S.autotune_classes(cands, reps=2)
+    tune_s = time.time() - t2
+    tuned_ms = timed(lambda: S.batch_solve_tuned(table))
+    equal = all(np.array_equal(a, r) for a, r in zip(S.batch_labels(), ref_labels))
+    out = {"groups": groups, "bind_s": time.time() - t0 - tune_s, "default_params_ms": by_class_ms,
+           "tuned_ms": tuned_ms, "tune_s": tune_s, "candidates": len(cands), "labels_equal_default": equal,
+           "table": {str(k): {"alpha": p.alpha, "beta": p.beta, "gamma": p.gamma, "theta": p.theta,
+                              "trim_times": p.trim_times, "wcc_mode": p.wcc_mode, "trim_scope": p.trim_scope,
+                              "switch_rule": p.switch_rule, "class_ms": table.ms[k]}
+                     for k, p in table.classes().items()}}
+    log(f"[batch] by size class: {groups} groups, default params {by_class_ms:.3f} ms, tuned {tuned_ms:.3f} ms "
+        f"(tuning {tune_s:.1f}s), labels equal {equal}")
     return out
 
 
@@ -274,6 +316,16 @@ def cpu_leg(dg, V: int, E: int, gpu_labels: np.ndarray, threads: int, runs: int,
                                         "avg_total_ms": O.parse_ref_average(r.stdout).get("Total time"),
                                         "runs": runs, "threads": threads, "labels_equal_gpu": ref_ok}
                     log(f"[cpu] reference: {out['reference']}")
+                    half = max(1, threads // 2)
+                    if half < threads:  # the scaling slope at this share
+                        env2 = dict(env, OMP_NUM_THREADS=str(half))
+                        r2 = subprocess.run([O.REF_BIN, *paths, str(half), "30", "200", "10", "0.1", str(runs)],
+                                            env=env2, capture_output=True, text=True, timeout=timeout)
+                        if r2.returncode == 0:
+                            t2 = [float(l.split(",")[1]) for l in r2.stdout.splitlines() if l.startswith("total time,")]
+                            out["reference_half_threads"] = {"threads": half, "ms_mean": statistics.fmean(t2),
+                                                             "ms_median": statistics.median(t2), "runs": runs}
+                            log(f"[cpu] reference at {half} threads: {out['reference_half_threads']}")
                 else:
                     out["reference_error"] = f"exit {r.returncode}"
         except subprocess.TimeoutExpired:
@@ -290,7 +342,8 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-runs", type=int, default=5)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: the process's CPU share (OMP_NUM_THREADS when set, else every CPU in its affinity set)")
     ap.add_argument("--timing", type=int, default=2, help="library timing level inside the timed region")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the autotuned measurement beside the main line")
@@ -421,8 +474,17 @@ def main(argv=None) -> int:
         if os.path.exists(pmc):
             with open(pmc) as f:
                 d = json.load(f)
-            result["roofline"]["traffic"] = d.get("hbm_bytes_per_launch")
+            tb = d.get("hbm_bytes_per_launch")
+            result["roofline"]["traffic"] = tb
             result["roofline"]["traffic_source"] = os.path.relpath(pmc, ROOT)
+            if tb:  # the physical position: counted HBM bytes per launch over the same launch time
+                gbps = tb / (avg_ms * 1e-3) / 1e9
+                result["roofline"]["physical"] = {"gbps": gbps, "frac": gbps / HBM_PEAK_GBS,
+                                                  "traffic_over_algorithmic": tb / per_launch_bytes}
+        # `achieved` counts every adjacency entry of the reached sets (the
+        # per-edge model): bottom-up levels read far fewer, so it is an
+        # edge-equivalent rate and can pass the peak
+        result["roofline"]["frac_above_peak"] = achieved > HBM_PEAK_GBS
     if a.batch_graphs > 0 or a.served_requests > 0:
         dg.free()
         dg = None
@@ -434,7 +496,8 @@ def main(argv=None) -> int:
     if not a.no_cpu:
         if dg is None:
             dg, _ = build_graph(S, a.workload, a.seed)
-        threads = a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        host = host_cpus()
+        threads = a.cpu_threads or host["share"]
         cpu = cpu_leg(dg, V, E, labels, threads, a.cpu_runs, timeout=600)
         ref = cpu.get("reference")
         base = ref if ref else cpu["port"]
@@ -445,7 +508,7 @@ def main(argv=None) -> int:
                       f"'total time' (wall clock around its parallel region)" if ref else
                       f"full {cfg['workload']} graph, {a.cpu_runs} solves of the oracle restatement",
             "ms_mean": base["ms_mean"], "ms_median": base["ms_median"],
-            "cpu_model": _cpu_model(), "detail": cpu}
+            "cpu_model": host["model"], "host": host, "detail": cpu}
         result["gpu_over_cpu"] = base["ms_mean"] / ms_per_step
         result["labels_equal_cpu"] = base["labels_equal_gpu"]
     if dg is not None:
@@ -473,15 +536,37 @@ def copy_calibration(nbytes: int, reps: int) -> int:
     return 0
 
 
-def _cpu_model() -> str:
+def host_cpus() -> dict:
+    """The host's CPUs as this process sees them: the model string, logical
+    CPUs, the affinity set, physical cores (distinct physical id / core id
+    pairs in /proc/cpuinfo), and the share a CPU baseline may use -- the
+    thread count the environment grants (OMP_NUM_THREADS), else the affinity
+    set."""
+    model, pairs = "unknown", set()
+    phys = core = None
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    pairs.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            pairs.add((phys, core))
     except OSError:
         pass
-    return "unknown"
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(omp) if omp.isdigit() and int(omp) > 0 else aff
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": aff,
+            "physical_cores": len(pairs) or None, "omp_num_threads": omp or None, "share": min(share, aff)}
 
 
 if __name__ == "__main__":

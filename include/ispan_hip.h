@@ -83,7 +83,13 @@ typedef struct ispan_params {
                              active vertex of the graph (trim_*_from_graph,
                              src/trim_2_3.h:9-144, 287-480, 682-879); engine 0
                              only.  Same detections, different work.          */
-    int32_t reserved[2];
+    int32_t batch_by_class; /* ispan_batch_bind_device: 1 orders the graphs by
+                             size class (see ispan_tuning_table) before packing
+                             them into groups, so a group spans a narrow range
+                             of sizes and is solved with the tuned parameters
+                             of its largest graph's class; 0 packs consecutive
+                             graphs                                           */
+    int32_t reserved[1];
 } ispan_params;
 
 /* Per-solve statistics.  ms[] uses the reference's avg_time slot map
@@ -233,6 +239,40 @@ const int32_t* ispan_batch_labels_device(const ispan_ctx* ctx);
 int ispan_autotune(ispan_ctx* ctx, const ispan_params* candidates, int32_t n, int32_t reps, int32_t* best,
                    double* ms);
 
+/* ---- tuning per size class ---------------------------------------------
+ * No counterpart in the reference, whose alpha/beta/gamma/theta come from
+ * argv (src/scc_main.cpp:20-25) for every graph alike.  A graph of V vertices
+ * is in size class floor(log2 V) (class 0 also takes V = 0); the table holds
+ * one parameter set per class.  Parameters choose the work, never the
+ * labels. */
+#define ISPAN_SIZE_CLASSES 32
+typedef struct ispan_tuning_table {
+    int32_t tuned[ISPAN_SIZE_CLASSES];       /* 1: params[k] was measured          */
+    ispan_params params[ISPAN_SIZE_CLASSES];
+    double ms[ISPAN_SIZE_CLASSES];           /* the choice's median time per class:
+                                                per graph for a single graph, per
+                                                pass over the class for a batch   */
+} ispan_tuning_table;
+
+/* Times every candidate on the bound graph (its class) or on each size class
+ * of the bound batch (a group counts for the class of its largest graph;
+ * bind with params->batch_by_class = 1 so that groups span narrow size
+ * ranges), `reps` times after one warm-up, and writes the fastest per
+ * measured class into table; other classes keep what table held. */
+int ispan_autotune_classes(ispan_ctx* ctx, const ispan_params* candidates, int32_t n, int32_t reps,
+                           ispan_tuning_table* table);
+/* ispan_batch_solve with each group solved by the parameters of its size
+ * class (a group's class: that of its largest graph); untuned classes use
+ * the defaults. */
+int ispan_batch_solve_tuned(ispan_ctx* ctx, const ispan_tuning_table* table, ispan_stats* stats);
+/* The parameters the table gives a graph of V vertices (defaults when its
+ * class is untuned). */
+void ispan_tuning_lookup(const ispan_tuning_table* table, int64_t V, ispan_params* out);
+/* A table as text (one line per tuned class) and back; ISPAN_ERR_IO when the
+ * file cannot be written, read or parsed. */
+int ispan_tuning_save(const ispan_tuning_table* table, const char* path);
+int ispan_tuning_load(const char* path, ispan_tuning_table* table);
+
 /* ---- serving --------------------------------------------------------------
  * A request loop over device-resident graphs with continuous batching; no
  * counterpart in the reference.  Registered graphs are appended once to an
@@ -302,6 +342,9 @@ int ispan_server_wait_device(ispan_server* srv, int64_t request_id, const int32_
 int ispan_server_release(ispan_server* srv, int64_t request_id);
 int ispan_server_stats_get(ispan_server* srv, ispan_server_stats* out);
 int ispan_server_reset_stats(ispan_server* srv);
+/* Solve each batch with the table's parameters for the size class of the
+ * batch's largest graph (NULL: back to the server's own parameters). */
+int ispan_server_set_tuning(ispan_server* srv, const ispan_tuning_table* table);
 const char* ispan_server_last_error(const ispan_server* srv);
 
 /* ---- reference file I/O and reporting ---------------------------------- */

@@ -42,7 +42,8 @@ EXPORTS = [
     "ispan_server_add_graph", "ispan_server_add_graph_device", "ispan_server_remove_graph", "ispan_server_submit", "ispan_server_submit_many",
     "ispan_server_wait", "ispan_server_wait_device", "ispan_server_release", "ispan_server_stats_get",
     "ispan_server_reset_stats", "ispan_server_last_error", "ispan_scc_detection_noalloc",
-    "ispan_scc_histogram",
+    "ispan_scc_histogram", "ispan_autotune_classes", "ispan_batch_solve_tuned", "ispan_tuning_lookup",
+    "ispan_tuning_save", "ispan_tuning_load", "ispan_server_set_tuning",
 ]
 
 
@@ -62,7 +63,33 @@ class Params(C.Structure):
                 ("trim_times", C.c_int32), ("thread_count", C.c_int32), ("pivot_wrap32", C.c_int32),
                 ("timing", C.c_int32), ("validate", C.c_int32), ("wcc_mode", C.c_int32),
                 ("async_batch", C.c_int32), ("engine", C.c_int32), ("switch_rule", C.c_int32),
-                ("trim_scope", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("trim_scope", C.c_int32), ("batch_by_class", C.c_int32), ("reserved", C.c_int32 * 1)]
+
+
+SIZE_CLASSES = 32
+
+
+class TuningTable(C.Structure):
+    """ispan_tuning_table: tuned parameters per size class floor(log2 V)."""
+    _fields_ = [("tuned", C.c_int32 * SIZE_CLASSES), ("params", Params * SIZE_CLASSES),
+                ("ms", C.c_double * SIZE_CLASSES)]
+
+    def classes(self) -> dict:
+        return {k: self.params[k] for k in range(SIZE_CLASSES) if self.tuned[k]}
+
+    def lookup(self, V: int) -> Params:
+        p = Params()
+        lib().ispan_tuning_lookup(C.byref(self), V, C.byref(p))
+        return p
+
+    def save(self, path: str):
+        _check(lib().ispan_tuning_save(C.byref(self), path.encode()))
+
+    @classmethod
+    def load(cls, path: str) -> "TuningTable":
+        t = cls()
+        _check(lib().ispan_tuning_load(path.encode(), C.byref(t)))
+        return t
 
 
 class Stats(C.Structure):
@@ -162,6 +189,12 @@ def lib():
         "ispan_scc_detection": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
                                           C.POINTER(Stats)]),
         "ispan_scc_histogram": (C.c_int, [_vp, _vp, C.c_int32, C.POINTER(Summary)]),
+        "ispan_autotune_classes": (C.c_int, [_vp, C.POINTER(Params), C.c_int32, C.c_int32, C.POINTER(TuningTable)]),
+        "ispan_batch_solve_tuned": (C.c_int, [_vp, C.POINTER(TuningTable), C.POINTER(Stats)]),
+        "ispan_tuning_lookup": (None, [C.POINTER(TuningTable), C.c_int64, C.POINTER(Params)]),
+        "ispan_tuning_save": (C.c_int, [C.POINTER(TuningTable), C.c_char_p]),
+        "ispan_tuning_load": (C.c_int, [C.c_char_p, C.POINTER(TuningTable)]),
+        "ispan_server_set_tuning": (C.c_int, [_vp, C.POINTER(TuningTable)]),
         "ispan_scc_detection_noalloc": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, C.POINTER(Params), _vp,
                                                   _vp, C.POINTER(Stats)]),
         "ispan_graph_load": (C.c_int, [C.c_char_p] * 4 + [C.POINTER(HostGraphC)]),
@@ -429,6 +462,22 @@ class Solver:
         _check(lib().ispan_autotune(self._ctx, arr, len(candidates), reps, C.byref(best), _ptr(ms)), self._ctx)
         return candidates[best.value], ms.tolist()
 
+    def autotune_classes(self, candidates: list[Params], reps: int = 3,
+                         table: "TuningTable | None" = None) -> "TuningTable":
+        """Fastest candidate per size class of the bound graph or batch, into
+        (a copy of) table."""
+        t = TuningTable()
+        if table is not None:
+            C.memmove(C.byref(t), C.byref(table), C.sizeof(TuningTable))
+        arr = (Params * len(candidates))(*candidates)
+        _check(lib().ispan_autotune_classes(self._ctx, arr, len(candidates), reps, C.byref(t)), self._ctx)
+        return t
+
+    def batch_solve_tuned(self, table: "TuningTable") -> Stats:
+        st = Stats()
+        _check(lib().ispan_batch_solve_tuned(self._ctx, C.byref(table), C.byref(st)), self._ctx)
+        return st
+
     def gen_rmat(self, V: int, edge_factor: float = 16.0, seed: int = 1, a=0.57, b=0.19, c=0.19) -> DeviceGraph:
         ptrs = [C.POINTER(C.c_int32)() for _ in range(4)]
         E = C.c_int64()
@@ -538,6 +587,10 @@ class Server:
     def reset_stats(self):
         self._err(lib().ispan_server_reset_stats(self._srv))
 
+    def set_tuning(self, table: "TuningTable | None"):
+        """Solve each batch with the table's parameters for its largest graph's size class."""
+        self._err(lib().ispan_server_set_tuning(self._srv, C.byref(table) if table is not None else None))
+
     def close(self):
         if self._srv:
             lib().ispan_server_destroy(self._srv)
@@ -618,6 +671,17 @@ class DeviceBatch:
 def tuning_grid(alphas=(4, 10, 30, 100), betas=(50, 200, 800), **kw) -> list[Params]:
     """Candidate parameter sets for Solver.autotune: every (alpha, beta) pair."""
     return [default_params(alpha=a, beta=b, **kw) for a in alphas for b in betas]
+
+
+def class_tuning_grid(alphas=(2, 10, 30), betas=(50, 200, 5000)) -> list[Params]:
+    """Candidates for Solver.autotune_classes: every (alpha, beta) pair, then
+    the defaults with one other knob changed at a time -- the reference's
+    frontier estimate, each small-SCC strategy, the full-graph pattern walk,
+    a longer trim-1 phase and another bottom-up level cap."""
+    c = tuning_grid(alphas, betas)
+    c += [default_params(switch_rule=0), default_params(wcc_mode=1), default_params(wcc_mode=2),
+          default_params(trim_scope=1), default_params(theta=0.01, trim_times=10), default_params(gamma=30)]
+    return c
 
 
 def scc_detection(g: HostGraph, alpha: int = 30, beta: int = 200, gamma: int = 10, theta: float = 0.1,

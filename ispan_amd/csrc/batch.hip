@@ -40,6 +40,19 @@ struct CatSrc {
     __device__ const int32_t* adj_of(int32_t g) const { return adj + e_off[g]; }
 };
 
+// Where graph i of a group is when the group takes graphs that are not
+// consecutive in the caller's layout (a group of one size class): the
+// caller's graph sg[i], whose offsets start at beg[sv[i] + sg[i]] and whose
+// adjacency starts at adj[se[i]] (sv, se: the caller's v_off, e_off of it).
+struct PermSrc {
+    const int32_t* beg;
+    const int32_t* adj;
+    const int64_t* sv;
+    const int64_t* se;
+    const int32_t* sg;
+    __device__ const int32_t* beg_of(int32_t i) const { return beg + sv[i] + sg[i]; }
+    __device__ const int32_t* adj_of(int32_t i) const { return adj + se[i]; }
+};
 
 // Union offsets: ubeg[u] for the group's vertices u = v_off[g] - v_off[g0] + i
 // is (e_off[g] - e_off[g0]) + beg_g[i].  Blocks take contiguous chunks, so a
@@ -265,6 +278,14 @@ hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, co
                         d_e_off, g0, ng, Vu, Eu, ufwb, ufwa, ubwb, ubwa, bad, s);
 }
 
+hipError_t launch_batch_union_perm(const int32_t* fwb_cat, const int32_t* fwa_cat, const int32_t* bwb_cat,
+                                   const int32_t* bwa_cat, const int64_t* d_lv_off, const int64_t* d_le_off,
+                                   const int64_t* d_sv, const int64_t* d_se, const int32_t* d_sg, int32_t ng,
+                                   int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb, int32_t* ubwa,
+                                   int32_t* bad, hipStream_t s) {
+    return union_launch(PermSrc{fwb_cat, fwa_cat, d_sv, d_se, d_sg}, PermSrc{bwb_cat, bwa_cat, d_sv, d_se, d_sg},
+                        d_lv_off, d_le_off, 0, ng, Vu, Eu, ufwb, ufwa, ubwb, ubwa, bad, s);
+}
 
 hipError_t launch_batch_roots(const int32_t* fwb, const int32_t* bwb, const BatchRanges& r, int32_t wrap32,
                               hipStream_t s) {

@@ -186,6 +186,15 @@ hipError_t launch_batch_union(const int32_t* fwb_cat, const int32_t* fwa_cat, co
                               const int32_t* bwa_cat, const int64_t* d_v_off, const int64_t* d_e_off, int32_t g0,
                               int32_t ng, int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb,
                               int32_t* ubwa, int32_t* bad, hipStream_t s);
+// The same for a group of graphs that are not consecutive in the caller's
+// layout: group graph i is the caller's graph sg[i] with the caller's offsets
+// sv[i] = v_off[sg[i]], se[i] = e_off[sg[i]]; lv_off / le_off are the
+// group's own prefix sums (ng + 1 entries from 0).
+hipError_t launch_batch_union_perm(const int32_t* fwb_cat, const int32_t* fwa_cat, const int32_t* bwb_cat,
+                                   const int32_t* bwa_cat, const int64_t* d_lv_off, const int64_t* d_le_off,
+                                   const int64_t* d_sv, const int64_t* d_se, const int32_t* d_sg, int32_t ng,
+                                   int64_t Vu, int64_t Eu, int32_t* ufwb, int32_t* ufwa, int32_t* ubwb, int32_t* ubwa,
+                                   int32_t* bad, hipStream_t s);
 // Graph ranges of one batch solve (device arrays, see batch.hip): graph g
 // owns vertices [vb[g], ve[g]) of the CSR being solved; its local labels go
 // to out[g]; cstart is the prefix of its kBatchChunk-vertex chunk counts.

@@ -91,6 +91,8 @@ struct ispan_server {
     ispan_ctx* ctx = nullptr;
     int device = 0;
     ispan_params params{};
+    bool tuned = false;               // batches take params by size class from `tuning`
+    ispan_tuning_table tuning{};
     int64_t budget = 0;
     int32_t max_graphs = 0;
     int64_t max_vertices = 0;
@@ -219,6 +221,7 @@ void worker_loop(ispan_server* S) {
         std::unordered_map<int64_t, int64_t> offset;  // graph -> offset in the label buffer
         Flight f;
         ispan::ArenaView av{};
+        ispan_params bp{};
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_work.wait(lk, [S] { return S->stop || !S->pending.empty(); });
@@ -269,13 +272,18 @@ void worker_loop(ispan_server* S) {
                 r.buf = b;
                 r.d_labels = lb.d + offset[r.graph];
             }
+            int64_t bigV = 0;
             for (const auto& kv : offset) {
                 const Graph& g = S->graphs[kv.first];
                 if (g.V == 0) continue;
                 vb.push_back(g.vb);
                 ve.push_back(g.vb + g.V);
                 out.push_back(lb.d + kv.second);
+                bigV = std::max<int64_t>(bigV, g.V);
             }
+            // parameters of the batch: by the size class of its largest graph
+            bp = S->params;
+            if (S->tuned) ispan_tuning_lookup(&S->tuning, bigV, &bp);
             av = ispan::ArenaView{S->csr[0], S->csr[1], S->csr[2], S->csr[3], S->first[0], S->first[1],
                                   (int32_t)S->usedV, S->usedE};
         }
@@ -283,7 +291,7 @@ void worker_loop(ispan_server* S) {
         int rc = ISPAN_OK;
         if (!vb.empty()) {
             ispan_stats st{};
-            rc = ispan::solve_arena(S->ctx, av, (int32_t)vb.size(), vb.data(), ve.data(), out.data(), &S->params, &st);
+            rc = ispan::solve_arena(S->ctx, av, (int32_t)vb.size(), vb.data(), ve.data(), out.data(), &bp, &st);
         }
         if (!rc && (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess ||
                     hipEventRecord(f.done, S->compute_stream) != hipSuccess))
@@ -696,6 +704,14 @@ int ispan_server_stats_get(ispan_server* S, ispan_server_stats* out) {
     out->mean_ms = S->latencies.empty() ? 0.0 : sum / (double)S->latencies.size();
     out->max_ms = mx;
     out->mean_batch_graphs = S->batches ? (double)S->batch_graphs_total / (double)S->batches : 0.0;
+    return ISPAN_OK;
+}
+
+int ispan_server_set_tuning(ispan_server* S, const ispan_tuning_table* table) {
+    if (!S) return ISPAN_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->tuned = table != nullptr;
+    if (table) S->tuning = *table;
     return ISPAN_OK;
 }
 

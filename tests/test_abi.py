@@ -114,3 +114,29 @@ def test_cli_usage_without_gpu():
     assert r.returncode == 255
     assert r.stdout.startswith("args = 3")
     assert "Usage:" in r.stdout
+
+
+def test_tuning_table_file_roundtrip(tmp_path):
+    """A per-size-class table survives save/load; lookup falls back to the
+    defaults for untuned classes; a malformed file is rejected."""
+    t = I.TuningTable()
+    for k, (a, b, w) in {9: (4, 5000, 2), 14: (2, 50, 1), 20: (30, 200, 0)}.items():
+        t.tuned[k] = 1
+        t.params[k] = I.default_params(alpha=a, beta=b, wcc_mode=w, trim_scope=k % 2, theta=0.05)
+        t.ms[k] = k / 3
+    path = str(tmp_path / "table.txt")
+    t.save(path)
+    u = I.TuningTable.load(path)
+    assert sorted(u.classes()) == [9, 14, 20]
+    for k in (9, 14, 20):
+        p, q = t.params[k], u.params[k]
+        assert (p.alpha, p.beta, p.wcc_mode, p.trim_scope, p.theta) == (q.alpha, q.beta, q.wcc_mode, q.trim_scope,
+                                                                      q.theta)
+        assert abs(u.ms[k] - k / 3) < 1e-5
+    assert u.lookup(700).alpha == 4 and u.lookup(1023).alpha == 4 and u.lookup(1024).alpha == 30
+    assert u.lookup(1 << 14).beta == 50 and u.lookup(5).alpha == 30  # class 2: untuned, defaults
+    bad = tmp_path / "bad.txt"
+    bad.write_text("9 4 5000\n")
+    with pytest.raises(I.IspanError) as e:
+        I.TuningTable.load(str(bad))
+    assert e.value.code == -7

@@ -174,3 +174,44 @@ def test_batch_c4_distribution(solver, max_group_edges):
     finally:
         for dg in dgs:
             dg.free()
+
+
+@pytest.mark.parametrize("cap", [0, 3000])
+def test_batch_by_class_and_tuned_solve(solver, cap):
+    """Binding by size class packs graphs in class order (scattered graphs
+    gathered into one union); per-class tuning fills the classes of the
+    groups' largest graphs; the tuned solve and the by-class solve give every
+    graph Tarjan's labels."""
+    graphs = mixed_graphs(21, 60) + [O.rmat(s, 8, 900 + s) for s in (12, 14, 9, 14, 12)]
+    ref = [O.tarjan(g) for g in graphs]
+    b = I.DeviceBatch.from_host([host(g) for g in graphs])
+    try:
+        solver.batch_bind(b, cap, I.default_params(batch_by_class=1))
+    finally:
+        b.free()
+    classes = {int(np.floor(np.log2(max(len(g.fw_beg) - 1, 1)))) for g in graphs}
+    assert (solver.batch_groups() == 1) == (cap == 0)
+    solver.batch_solve()
+    for lab, r in zip(solver.batch_labels(), ref):
+        assert np.array_equal(lab, r)
+    cands = [I.default_params(), I.default_params(alpha=2, beta=5000), I.default_params(wcc_mode=2),
+             I.default_params(wcc_mode=1, trim_scope=1)]
+    table = solver.autotune_classes(cands, reps=1)
+    tuned = set(table.classes())
+    assert tuned <= classes and max(classes) in tuned and (len(tuned) == 1) == (cap == 0)
+    st = solver.batch_solve_tuned(table)
+    assert st.ms[3] > 0
+    for lab, r in zip(solver.batch_labels(), ref):
+        assert np.array_equal(lab, r)
+
+
+def test_autotune_classes_single_graph(solver):
+    g = O.rmat(13, 8, 3)
+    solver.upload(host(g))
+    prior = I.TuningTable()
+    prior.tuned[5] = 1
+    prior.params[5] = I.default_params(alpha=7)
+    t = solver.autotune_classes([I.default_params(), I.default_params(wcc_mode=2, alpha=4)], reps=2, table=prior)
+    assert set(t.classes()) == {5, 13} and t.params[5].alpha == 7 and t.ms[13] > 0
+    solver.solve(t.lookup(g.V))
+    assert np.array_equal(solver.labels(), O.tarjan(g))

@@ -278,3 +278,23 @@ def test_free_memory_caps_admission(solver):
         while hog:
             hiprt.free(hog.pop())
         dg.free()
+
+
+def test_server_with_tuning_table(pool):
+    """Batches solved by the table's parameters for their largest graph's
+    size class give the same labels."""
+    graphs, ref = pool
+    t = I.TuningTable()
+    for k in range(I.SIZE_CLASSES):
+        t.tuned[k] = 1
+        t.params[k] = I.default_params(alpha=2 + k % 3, beta=[50, 5000][k % 2], wcc_mode=k % 3, trim_scope=k % 2)
+    with I.Server(0) as srv:
+        srv.set_tuning(t)
+        ids = [srv.add_graph(host(g)) for g in graphs]
+        rids = srv.submit_many(ids) + [srv.submit(gid) for gid in ids]
+        for k, rid in enumerate(rids):
+            lab, info = srv.wait(rid)
+            assert info.status == 0 and np.array_equal(lab, ref[k % len(ids)])
+        srv.set_tuning(None)
+        lab, _ = srv.wait(srv.submit(ids[3]))
+        assert np.array_equal(lab, ref[3])
