@@ -89,7 +89,13 @@ typedef struct ispan_params {
                              of sizes and is solved with the tuned parameters
                              of its largest graph's class; 0 packs consecutive
                              graphs                                           */
-    int32_t reserved[1];
+    int32_t td_local;     /* engine 0, top-down to exhaustion (after bottom-up):
+                             a wave expands what it claims itself, in the same
+                             level, for up to this many steps of 64 vertices
+                             before handing the rest to the next level (the
+                             reference's asynchronous top-down, src/fw_bw.h:
+                             506-544); 0 keeps every level synchronous.
+                             Default 64.  Chooses the work, never the labels.  */
 } ispan_params;
 
 /* Per-solve statistics.  ms[] uses the reference's avg_time slot map
@@ -123,7 +129,8 @@ typedef struct ispan_stats {
 typedef struct ispan_ctx ispan_ctx;
 
 /* Defaults: alpha 30, beta 200, gamma 10, theta 0.1, trim_times 3
- * (src/bash_one.sh:3, src/util.h:34), validate 1, everything else 0. */
+ * (src/bash_one.sh:3, src/util.h:34), validate 1, async_batch 8,
+ * switch_rule 1, td_local 64, everything else 0. */
 void ispan_default_params(ispan_params* p);
 
 /* Human-readable text for an error code. */

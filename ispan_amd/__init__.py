@@ -63,7 +63,7 @@ class Params(C.Structure):
                 ("trim_times", C.c_int32), ("thread_count", C.c_int32), ("pivot_wrap32", C.c_int32),
                 ("timing", C.c_int32), ("validate", C.c_int32), ("wcc_mode", C.c_int32),
                 ("async_batch", C.c_int32), ("engine", C.c_int32), ("switch_rule", C.c_int32),
-                ("trim_scope", C.c_int32), ("batch_by_class", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("trim_scope", C.c_int32), ("batch_by_class", C.c_int32), ("td_local", C.c_int32)]
 
 
 SIZE_CLASSES = 32
@@ -677,10 +677,12 @@ def class_tuning_grid(alphas=(2, 10, 30), betas=(50, 200, 5000)) -> list[Params]
     """Candidates for Solver.autotune_classes: every (alpha, beta) pair, then
     the defaults with one other knob changed at a time -- the reference's
     frontier estimate, each small-SCC strategy, the full-graph pattern walk,
-    a longer trim-1 phase and another bottom-up level cap."""
+    a longer trim-1 phase, another bottom-up level cap, and in-place
+    expansion off or with a larger step budget."""
     c = tuning_grid(alphas, betas)
     c += [default_params(switch_rule=0), default_params(wcc_mode=1), default_params(wcc_mode=2),
-          default_params(trim_scope=1), default_params(theta=0.01, trim_times=10), default_params(gamma=30)]
+          default_params(trim_scope=1), default_params(theta=0.01, trim_times=10), default_params(gamma=30),
+          default_params(td_local=0), default_params(td_local=1024)]
     return c
 
 

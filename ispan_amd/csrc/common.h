@@ -160,6 +160,7 @@ struct GiantArgs {
     int32_t wcc_mode;         // after the trims: 0 union-find WCC, 1 label-propagation WCC, 2 the
                               // colouring SCC method (no mice phase left)
     int32_t trim_scope;       // size-2/3 pattern walk: 0 the active queue, 1 every active vertex
+    int32_t td_local;         // top-down to exhaustion: steps a wave expands its own finds in place (0: off)
     uint64_t* ckey;           // colouring: (priority, colour) per vertex
     int32_t* aux;             // colouring: smallest member per SCC root
 };

@@ -452,7 +452,88 @@ __device__ __forceinline__ void push_found_n(const bool (&claim)[K], const int32
     }
 }
 
-// Top-down: edges per lane per step.  One measured fastest: two let their
+// Top-down to exhaustion (mode 2) without level barriers in the thin parts:
+// a wave expands the vertices it claimed itself, in the same level, instead
+// of publishing them for the next level -- the reference's asynchronous
+// top-down (src/fw_bw.h:506-544, 1068-1106), where each thread keeps
+// expanding its own circular queue.  The wave's LDS staging buffer is that
+// queue: up to kWave pending entries per step are flattened (wave-level
+// degree scan) and walked one edge per lane; new claims of degree below
+// kGiantHubDeg go back into the queue, hubs become next-level work items,
+// and when the queue is full -- or after max_steps steps, so that a region
+// one wave holds is shared again -- entries go to the next level's queue.
+// Claims stay exact (the same wave-combined atomics), so the reach set and
+// the found counts are those of a level-synchronous traversal; only the
+// level at which a vertex is expanded changes.  All lanes call it.
+template <bool BW>
+__device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root, const int32_t* __restrict__ beg,
+                                             const int32_t* __restrict__ adj, const uint64_t* __restrict__ allow,
+                                             uint64_t* vis, int32_t* qout, int32_t* qn_out, HubStage& hs,
+                                             HubItem* hout, int32_t* hn_out, int64_t& my_found, int32_t& gmin,
+                                             int max_steps) {
+    int32_t* off = c.sm.off + c.wave * kWave;  // this wave's slice of the tile arrays
+    int32_t* stv = c.sm.start + c.wave * kWave;
+    int32_t* buf = wq.buf;
+    int head = 0, n = wq.n;  // pending: buf[head .. n) modulo kWaveBuf
+    wave_sync();
+    for (int step = 0; n > head && step < max_steps; ++step) {
+        const int k = min(kWave, n - head);
+        int32_t st = 0, dg = 0;
+        if (c.lane < k) {
+            const int32_t v = buf[(head + c.lane) % kWaveBuf];
+            st = beg[v];
+            dg = beg[v + 1] - st;
+        }
+        head += k;
+        const int incl = wave_incl_scan(dg);
+        const int total = lane_get(incl, kWave - 1);
+        off[c.lane] = incl - dg;
+        stv[c.lane] = st;
+        wave_sync();
+        for (int eb = 0; eb < total; eb += kWave) {
+            // the source of edge e: the last j < k with off[j] <= e
+            const int e = eb + c.lane;
+            int pos = 0;
+#pragma unroll
+            for (int sp = kWave / 2; sp > 0; sp >>= 1)
+                if (pos + sp < k && off[pos + sp] <= e) pos += sp;
+            bool want[1];
+            int32_t w[1];
+            w[0] = e < total ? adj[stv[pos] + (e - off[pos])] : 0;
+            want[0] = e < total && w[0] != root;
+            bool claim[1];
+            int32_t s0[1], d[1];
+            claim_with_degree<1>(want, w, allow, vis, beg, claim, s0, d);
+            my_found += claim[0];
+            if (BW && claim[0]) gmin = min(gmin, w[0]);
+            hub_push(hs, claim[0] && d[0] >= kGiantHubDeg, w[0], s0[0], d[0], hout, hn_out);
+            const bool keep = claim[0] && d[0] > 0 && d[0] < kGiantHubDeg;
+            const uint64_t m = __ballot(keep);
+            const int cnt = __popcll(m);
+            if (cnt && n - head + cnt <= kWaveBuf) {
+                if (keep) buf[(n + __popcll(m & lanemask_lt())) % kWaveBuf] = w[0];
+                n += cnt;
+            } else if (cnt) {  // the queue is full: these go to the next level
+                int base = 0;
+                if (c.lane == 0) base = atomicAdd(qn_out, cnt);
+                base = lane_get(base, 0);
+                if (keep) st_agent(&qout[base + __popcll(m & lanemask_lt())], w[0]);
+            }
+        }
+        wave_sync();  // the queue entries and the slice are rewritten next step
+    }
+    const int left = n - head;  // the step budget ran out: the next level takes them
+    if (left > 0) {
+        int base = 0;
+        if (c.lane == 0) base = atomicAdd(qn_out, left);
+        base = lane_get(base, 0);
+        for (int i = c.lane; i < left; i += kWave) st_agent(&qout[base + i], buf[(head + i) % kWaveBuf]);
+    }
+    wq.n = 0;
+    wave_sync();
+}
+
+// Top-down: edges per lane per step.// Top-down: edges per lane per step.  One measured fastest: two let their
 // load chains overlap, but the registers they hold push the kernel into
 // spills on other paths.
 constexpr int kTK = 1;
@@ -634,6 +715,13 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     }
                 }
             }
+            // in place, in top-down to exhaustion only: where the frontier
+            // starts small (the start of a traversal), a few waves would
+            // expand a growing region alone (measured: 4096^2 grid 5.09 ->
+            // 5.8-6.5 ms with levels of up to 256-65536 entries expanded so)
+            if (mode == 2 && a.td_local > 0)
+                expand_local<BW>(c, wq, root, beg, adj, allow, vis, qout, qn_out, hs, hout, hn_out, my_found, gmin,
+                                 a.td_local);
             block_flush_both(c, wq, qout, qn_out, hs, hout, hn_out);
         } else {
             // ---- bottom-up over the candidate bitmap, one wave per group of
@@ -870,6 +958,11 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
         ++L;
         if (found == 0) {
             block_acquire();  // what follows reads the visited bits exactly
+            break;
+        }
+        if (topdown && a.td_local > 0 &&  // the waves may have expanded everything they found
+            load_agent(&gs->qn[dir][L % 3].v) == 0 && load_agent(&gs->hn[dir][L % 3].v) == 0) {
+            block_acquire();
             break;
         }
         if (L > (int64_t)a.V + 2) {

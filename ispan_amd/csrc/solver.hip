@@ -476,6 +476,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.bu_mult = giant_bu_mult(ctx->coop_grid);
         ga.wcc_mode = P->wcc_mode;
         ga.trim_scope = P->trim_scope;
+        ga.td_local = std::max(P->td_local, 0);
         ga.ckey = ctx->keys;
         ga.aux = ctx->qbuf;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
@@ -861,6 +862,7 @@ void ispan_default_params(ispan_params* p) {
     p->async_batch = 8;
     p->engine = 0;
     p->switch_rule = 1;
+    p->td_local = 64;
 }
 
 const char* ispan_strerror(int code) {

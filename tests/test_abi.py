@@ -46,6 +46,7 @@ def test_library_is_gfx950():
 def test_default_params_and_strerror():
     p = I.default_params()
     assert (p.alpha, p.beta, p.gamma, p.trim_times, p.validate) == (30, 200, 10, 3, 1)
+    assert (p.switch_rule, p.td_local, p.wcc_mode, p.trim_scope, p.engine) == (1, 64, 0, 0, 0)
     assert abs(p.theta - 0.1) < 1e-12
     L = I.lib()
     for code, name in I.ERRORS.items():

@@ -59,7 +59,8 @@ PARAMS = [dict(), dict(wcc_mode=1), dict(pivot_wrap32=1), dict(alpha=1, beta=1, 
           dict(alpha=1000000, beta=2, gamma=1), dict(alpha=2, beta=1000000, gamma=100, async_batch=1),
           dict(theta=0.9, trim_times=0), dict(theta=0.0, trim_times=5), dict(wcc_mode=2), dict(trim_scope=1),
           dict(wcc_mode=2, trim_scope=1, theta=0.9, trim_times=0), dict(wcc_mode=1, engine=1), dict(switch_rule=0),
-          dict(wcc_mode=1, trim_times=0)]
+          dict(wcc_mode=1, trim_times=0), dict(alpha=1, beta=1, gamma=0, td_local=4),
+          dict(alpha=1, beta=1, gamma=0, td_local=1000), dict(td_local=16)]
 
 
 @pytest.mark.parametrize("seed", range(96))
@@ -538,5 +539,25 @@ def test_small_scc_arms_hashed(solver, key, wcc_mode, trim_scope):
             assert st.host_syncs <= 2 and st.mice == 0
         elif st.mice > 0:
             assert st.n_wcc > 0
+    finally:
+        dg.free()
+
+
+@pytest.mark.parametrize("key", ["rmat-20-16-seed1", "rmat-24-16-seed1", "grid-1024-0.6-seed1", "grid-4096-0.6-seed1"])
+@pytest.mark.parametrize("td_local", [1, 16, 64, 1000])
+def test_local_expansion_hashed(solver, key, td_local):
+    """Top-down to exhaustion with waves expanding their own finds in place
+    (td_local steps) reaches the same sets: golden labels, and the same FW
+    reach and giant sizes as the level-synchronous traversal."""
+    d = dict(G.hash_cases())[key]
+    dg = solver.gen_rmat(1 << d["size"], float(d["param"]), d["seed"]) if d["kind"] == "rmat" else \
+        solver.gen_grid(d["size"], float(d["param"]), d["seed"])
+    try:
+        solver.bind(dg)
+        s0 = solver.solve(I.default_params(td_local=0))  # every level synchronous
+        st = solver.solve(I.default_params(td_local=td_local))
+        assert G.sha(solver.labels()) == d["labels_sha256"]
+        assert (st.fw_size, st.giant_size) == (s0.fw_size, s0.giant_size)
+        assert st.fw_levels <= s0.fw_levels and st.bw_levels <= s0.bw_levels
     finally:
         dg.free()
