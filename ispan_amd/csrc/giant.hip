@@ -533,6 +533,10 @@ __device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root
     wave_sync();
 }
 
+// Level-synchronous top-down levels are expanded in place (as in mode 2)
+// once their input gives every wave at least this many vertices.
+constexpr int kLocalPerWave = 16;
+
 // Top-down: edges per lane per step.// Top-down: edges per lane per step.  One measured fastest: two let their
 // load chains overlap, but the registers they hold push the kernel into
 // spills on other paths.
@@ -715,11 +719,13 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
                     }
                 }
             }
-            // in place, in top-down to exhaustion only: where the frontier
-            // starts small (the start of a traversal), a few waves would
-            // expand a growing region alone (measured: 4096^2 grid 5.09 ->
-            // 5.8-6.5 ms with levels of up to 256-65536 entries expanded so)
-            if (mode == 2 && a.td_local > 0)
+            // in place: in top-down to exhaustion, and in any later top-down
+            // level whose input gives every wave at least kLocalPerWave
+            // vertices.  Where the frontier is small (the start of a
+            // traversal on a high-diameter graph) a few waves would expand a
+            // growing region alone: levels of up to 256-65536 entries
+            // expanded so took the 4096^2 grid from 5.09 to 5.8-6.5 ms.
+            if (a.td_local > 0 && (mode == 2 || (L > 0 && nh == 0 && n >= kLocalPerWave * total_waves)))
                 expand_local<BW>(c, wq, root, beg, adj, allow, vis, qout, qn_out, hs, hout, hn_out, my_found, gmin,
                                  a.td_local);
             block_flush_both(c, wq, qout, qn_out, hs, hout, hn_out);
