@@ -227,10 +227,14 @@ def test_large_top_down_frontiers(solver, engine):
     # uniform random digraph, 10^6 vertices, out-degree 4: with alpha = 1 the
     # forward frontier grows 1 -> 11 -> ... -> 42 K -> 148 K -> 355 K -> 333 K
     # top-down before the switch, so its levels stage queue tiles of 64, 256
-    # and 1024 entries
+    # and 1024 entries (td_local = 0: every level synchronous)
     g = uniform_digraph(1_000_000, 4_000_000, 11)
-    st = check(solver, g, engine=engine, alpha=1)
+    st = check(solver, g, engine=engine, alpha=1, td_local=0)
     assert st.fw_levels >= 10 and st.giant_size > 900_000
+    # with in-place expansion the large levels are walked by the waves that
+    # found them: fewer levels, the same labels
+    st2 = check(solver, g, engine=engine, alpha=1)
+    assert st2.fw_levels <= st.fw_levels and st2.giant_size == st.giant_size
 
 
 @pytest.mark.parametrize("engine", [0, 1])
