@@ -123,7 +123,11 @@ typedef struct ispan_stats {
     int64_t n_scc;        /* number of SCCs (computed on request)            */
     int64_t host_syncs;   /* host<->device round trips in this solve         */
     int64_t engine_used;  /* giant-phase engine that ran (see ispan_params)  */
-    int64_t reserved[8];
+    int64_t bw_reach;     /* vertices the backward traversal reached: engine
+                             0 runs it beside the forward one over every
+                             active vertex (the giant is the intersection);
+                             engine 1 inside the forward set (= giant_size) */
+    int64_t reserved[7];
 } ispan_stats;
 
 typedef struct ispan_ctx ispan_ctx;

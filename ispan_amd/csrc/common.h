@@ -80,12 +80,14 @@ struct GiantState {
     GLine err;          // 1 barrier timeout, 2 level cap, 4 queue count out of range
     int64_t root, fq0, fw_count, giant, gmin, fq_n, fw_levels, bw_levels;
     int64_t work[4];    // adjacency entries read top-down / bottom-up, vertices found top-down / bottom-up
-    int64_t ts[6];      // wall-clock stamps: start, pivot known, FW done, BW done, giant relabelled, end
+    int64_t ts[6];      // wall-clock stamps: start, pivot known, FW done, BW done (the two run in the
+                        // same levels), giant relabelled, end
     GLine fqn[3];       // active-queue lengths of the in-kernel trim sweeps (rotating)
     int64_t n_final, final_sel, trim1, pat2, pat3, sweeps, wcc_done;
     int64_t tail_rounds;  // label-propagation sweeps (wcc_mode 1) or colouring rounds (2)
     int64_t tail_sccs;    // colouring: SCCs it labelled
-    int64_t pad[3];
+    int64_t bw_reach;     // vertices the backward traversal reached (the giant and what reaches it)
+    int64_t pad[2];
     RLine gacc[32];     // barrier reductions: per arrival group
     RLine tacc;         // barrier reductions: over groups
     RLine res[2];       // barrier reductions: result of generation g in res[g & 1]
@@ -124,11 +126,16 @@ struct GiantArgs {
     uint64_t* act;
     uint64_t* fvis;
     uint64_t* bvis;
-    uint64_t* snap;
-    int32_t* q0;
+    uint64_t* snap;           // forward bottom-up: visited words before the level
+    uint64_t* snapb;          // backward
+    int32_t* q0;              // forward top-down queues (ping-pong by level)
     int32_t* q1;
-    HubItem* h0;
+    HubItem* h0;              // forward hub work items
     HubItem* h1;
+    int32_t* qb0;             // backward queues and hub work items
+    int32_t* qb1;
+    HubItem* hb0;
+    HubItem* hb1;
     int32_t* fq;              // active queue after the giant phase
     GiantState* gs;
     int64_t* part;            // kGiantSlots x grid partial counts

@@ -27,8 +27,8 @@ constexpr int kGBlock = 1024;
 constexpr int kW = kGBlock / kWave;
 
 // Partial-count slots: part[slot * grid + block].
-enum Slot { kPivP, kPivI, kSurv, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU };
-static_assert(kFoundBU < kGiantSlots, "slot count");
+enum Slot { kPivP, kPivI, kSurv, kPost, kEdgeTD, kEdgeBU, kFoundTD, kFoundBU, kGiantN };
+static_assert(kGiantN < kGiantSlots, "slot count");
 
 struct Smem {
     int32_t off[kGBlock];    // top-down tile: exclusive degree prefix, first edge
@@ -469,8 +469,7 @@ template <bool BW>
 __device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root, const int32_t* __restrict__ beg,
                                              const int32_t* __restrict__ adj, const uint64_t* __restrict__ allow,
                                              uint64_t* vis, int32_t* qout, int32_t* qn_out, HubStage& hs,
-                                             HubItem* hout, int32_t* hn_out, int64_t& my_found, int32_t& gmin,
-                                             int max_steps) {
+                                             HubItem* hout, int32_t* hn_out, int64_t& my_found, int max_steps) {
     int32_t* off = c.sm.off + c.wave * kWave;  // this wave's slice of the tile arrays
     int32_t* stv = c.sm.start + c.wave * kWave;
     int32_t* buf = wq.buf;
@@ -505,7 +504,6 @@ __device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root
             int32_t s0[1], d[1];
             claim_with_degree<1>(want, w, allow, vis, beg, claim, s0, d);
             my_found += claim[0];
-            if (BW && claim[0]) gmin = min(gmin, w[0]);
             hub_push(hs, claim[0] && d[0] >= kGiantHubDeg, w[0], s0[0], d[0], hout, hn_out);
             const bool keep = claim[0] && d[0] > 0 && d[0] < kGiantHubDeg;
             const uint64_t m = __ballot(keep);
@@ -542,13 +540,27 @@ constexpr int kLocalPerWave = 16;
 // spills on other paths.
 constexpr int kTK = 1;
 
-// One direction of the traversal.  Returns the number of vertices claimed
-// (the roots excluded), or -1 after a failed barrier.  BW: claims are limited
-// to the forward set and *gmin receives the smallest id reached.  With
-// a.n_roots > 0 every listed root seeds level 0 (*seeded_out of them).
+// Per-direction state of the joint traversal (the same in every block).
+struct Dir {
+    int mode;         // 0 top-down, 1 bottom-up, 2 top-down to exhaustion
+    int64_t L;        // levels done
+    int64_t visited;  // vertices claimed, the roots excluded
+    bool live;        // still traversing
+    bool gather;      // its last level was bottom-up and its next is top-down
+};
+
+// One level of one direction: top-down over the level's queue and hub items
+// (then, where it applies, in place), or bottom-up over the candidate
+// bitmap.  my_found (per thread) counts the vertices claimed; fedges, the
+// out-degree sum of what was pushed for the next level (top-down) or the
+// adjacency entries scanned (bottom-up).  Forward: from the pivot along
+// out-edges, claiming active vertices into fvis.  Backward: along in-edges,
+// claiming active vertices into bvis, not only those of the forward set: the
+// giant SCC is fvis & bvis either way (a path from v to the pivot lies in
+// the forward set when v does), and the two directions then need not wait
+// for each other.
 template <bool BW>
-__device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_size, int64_t* levels_out,
-                                            int32_t* gmin_out, int64_t* seeded_out) {
+__device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, int64_t& my_found, int64_t& fedges) {
     const GiantArgs& a = c.a;
     Smem& sm = c.sm;
     const int32_t* beg = BW ? a.bwb : a.fwb;  // expansion (top-down) direction
@@ -556,463 +568,516 @@ __device__ __forceinline__ int64_t traverse(Ctx& c, int32_t root, int64_t set_si
     const int32_t* rbeg = BW ? a.fwb : a.bwb;  // bottom-up scans the reverse direction
     const int32_t* radj = BW ? a.fwa : a.bwa;
     const int32_t* rfirst = BW ? a.fw_first : a.bw_first;
-    const uint64_t* allow = BW ? a.fvis : a.act;
+    const uint64_t* allow = a.act;
     uint64_t* vis = BW ? a.bvis : a.fvis;
+    uint64_t* snap = BW ? a.snapb : a.snap;
+    int32_t* q0 = BW ? a.qb0 : a.q0;
+    int32_t* q1 = BW ? a.qb1 : a.q1;
+    HubItem* h0 = BW ? a.hb0 : a.h0;
+    HubItem* h1 = BW ? a.hb1 : a.h1;
     GiantState* gs = a.gs;
     const int dir = BW ? 1 : 0;
     WaveQueue wq{sm.wbuf[c.wave], 0};
     HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
-
-    int64_t visited = 0;
-    int32_t gmin = BW ? root : INT32_MAX;
-    int mode = 0;  // 0 top-down, 1 bottom-up, 2 top-down to exhaustion
-    int64_t L = 0;
+    const int mode = d.mode;
+    const int64_t L = d.L;
     const int64_t total_waves = (int64_t)c.G * kW;
     const int64_t gwave = (int64_t)c.b * kW + c.wave;
     auto is_vis = [vis](int32_t u, int32_t) { return bit_of(vis, u); };
     const bool multi = a.n_roots > 0;
-
-    int64_t fedges = 0;  // per thread: out-degree sum of what this thread pushed for the next level
-    *seeded_out = 1;
-    if (multi) {
-        // level-0 input: every graph's root, marked visited, queued or split
-        // into hub items like any claimed vertex
-        int64_t seeded = 0;
-        for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < a.n_roots; ib += (int64_t)c.G * kGBlock) {
-            const int64_t i = ib + c.lane;
-            const int32_t r = i < a.n_roots ? a.roots[i] : -1;
-            const bool ok = r >= 0 && r < a.V && (!BW || bit_of(a.fvis, r));
-            if (ok) atomicOr(reinterpret_cast<unsigned long long*>(vis + (r >> 6)), 1ull << (r & 63));
-            seeded += ok;
-            push_next(ok, r, beg, wq, a.q0, &gs->qn[dir][0].v, hs, a.h0, &gs->hn[dir][0].v, fedges);
-        }
-        block_flush_both(c, wq, a.q0, &gs->qn[dir][0].v, hs, a.h0, &gs->hn[dir][0].v);
-        Red r;
-        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{block_sum(seeded, sm), 0, 0, 0}, &r, 1)) return -1;
-        *seeded_out = r.s0;
-    }
-
-    while (true) {
-        const int32_t* qin = (L & 1) ? a.q1 : a.q0;
-        int32_t* qout = (L & 1) ? a.q0 : a.q1;
-        const HubItem* hin = (L & 1) ? a.h1 : a.h0;
-        HubItem* hout = (L & 1) ? a.h0 : a.h1;
-        int32_t* qn_out = &gs->qn[dir][(L + 1) % 3].v;
-        int32_t* hn_out = &gs->hn[dir][(L + 1) % 3].v;
-        const bool topdown = mode != 1;
-        const int64_t edges0 = c.edges_td + c.edges_bu;  // thread 0's are the block's
-        const bool tr = a.trace && L < kLevelTrace;
-        const int64_t tl0 = tr ? (int64_t)wall_clock64() : 0;
-        fedges = 0;
-        int64_t my_found = 0;  // per thread
-        if (topdown) {
-            // ---- top-down: tiles of the queue, then hub work items; level 0
-            // expands the root's list in kGiantChunk-edge pieces.
-            int64_t n = 0, nh = 0;
-            int32_t rchunk = kGiantChunk;  // level 0: edges per piece of the root's list
-            if (L == 0 && !multi) {
-                if (c.b == 0 && c.tid == 0)
-                    atomicOr(reinterpret_cast<unsigned long long*>(vis + (root >> 6)), 1ull << (root & 63));
-                // the root's list in about one piece per block
-                const int32_t rdeg = beg[root + 1] - beg[root];
-                rchunk = max(256, ((rdeg + c.G - 1) / c.G + 63) & ~63);
-                rchunk = min(rchunk, kGiantChunk);
-                nh = (rdeg + rchunk - 1) / rchunk;
-            } else {
-                n = load_agent(&gs->qn[dir][L % 3].v);
-                nh = load_agent(&gs->hn[dir][L % 3].v);
-                if (n < 0 || n > a.V || nh < 0 || nh > a.hub_cap) {  // cannot happen with exact claims
-                    if (c.tid == 0) atomicOr(&gs->err.v, 4);
-                    n = 0;
-                    nh = 0;
-                }
-            }
-            static_assert((kGBlock & (kGBlock - 1)) == 0, "tile size");
-            // entries per tile: a power of two from 64 to kGBlock, the
-            // smallest that needs no more tiles than blocks, so a small
-            // frontier is spread over many blocks
-            int tile = kGBlock;
-            while (tile > kWave && (int64_t)(tile / 2) * c.G >= n) tile /= 2;
-            const int64_t ntiles = (n + tile - 1) / tile;
-            for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
-                if (u < ntiles) {
-                    // a tile: queue entries on threads 0..tile-1
-                    const int64_t i = u * tile + c.tid;
-                    int32_t st = 0, dg = 0;
-                    if (c.tid < tile && i < n) {
-                        const int32_t v = load_agent(&qin[i]);
-                        st = beg[v];
-                        dg = beg[v + 1] - st;
-                    }
-                    int excl, total;
-                    block_excl_scan<kW>(dg, excl, total, sm.tmp);
-                    sm.off[c.tid] = excl;
-                    sm.start[c.tid] = st;
-                    if (c.tid == 0) c.edges_td += total;
-                    __syncthreads();
-                    for (int eb = 0; eb < total; eb += kGBlock * kTK) {
-                        // lane owns edges eb + k * kGBlock + tid; the source of
-                        // edge e is the last j with off[j] <= e (off[0] = 0,
-                        // non-decreasing), found by a fixed-depth search
-                        int e[kTK], pos[kTK];
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) {
-                            e[k] = eb + k * kGBlock + c.tid;
-                            pos[k] = 0;
-                        }
-#pragma unroll
-                        for (int sp = kGBlock / 2; sp > 0; sp >>= 1) {
-#pragma unroll
-                            for (int k = 0; k < kTK; ++k)
-                                if (sm.off[pos[k] + sp] <= e[k]) pos[k] += sp;
-                        }
-                        bool want[kTK];
-                        int32_t w[kTK];
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) {
-                            w[k] = 0;
-                            if (e[k] < total) w[k] = adj[sm.start[pos[k]] + (e[k] - sm.off[pos[k]])];
-                        }
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) want[k] = e[k] < total && w[k] != root;
-                        bool claim[kTK];
-                        int32_t s0[kTK], d[kTK];
-                        claim_with_degree<kTK>(want, w, allow, vis, beg, claim, s0, d);
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) {
-                            my_found += claim[k];
-                            if (BW && claim[k]) gmin = min(gmin, w[k]);
-                        }
-                        push_found_n<kTK>(claim, w, s0, d, wq, qout, qn_out, hs, hout, hn_out, fedges);
-                    }
-                    __syncthreads();
-                } else {
-                    int32_t v, s0;
-                    if (L == 0 && !multi) {
-                        v = root;
-                        s0 = beg[root] + (int32_t)(u - ntiles) * rchunk;
-                    } else {
-                        v = load_agent(&hin[u - ntiles].v);
-                        s0 = load_agent(&hin[u - ntiles].start);
-                    }
-                    const int32_t e1 = min(s0 + (L == 0 && !multi ? rchunk : kGiantChunk), beg[v + 1]);
-                    if (c.tid == 0) c.edges_td += e1 - s0;
-                    for (int32_t eb = s0; eb < e1; eb += kGBlock * kTK) {
-                        bool want[kTK];
-                        int32_t w[kTK];
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) {
-                            const int32_t e = eb + k * kGBlock + c.tid;
-                            w[k] = e < e1 ? adj[e] : 0;
-                        }
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) want[k] = eb + k * kGBlock + c.tid < e1 && w[k] != root;
-                        bool claim[kTK];
-                        int32_t ws[kTK], wd[kTK];
-                        claim_with_degree<kTK>(want, w, allow, vis, beg, claim, ws, wd);
-#pragma unroll
-                        for (int k = 0; k < kTK; ++k) {
-                            my_found += claim[k];
-                            if (BW && claim[k]) gmin = min(gmin, w[k]);
-                        }
-                        push_found_n<kTK>(claim, w, ws, wd, wq, qout, qn_out, hs, hout, hn_out, fedges);
-                    }
-                }
-            }
-            // in place: in top-down to exhaustion, and in any later top-down
-            // level whose input gives every wave at least kLocalPerWave
-            // vertices.  Where the frontier is small (the start of a
-            // traversal on a high-diameter graph) a few waves would expand a
-            // growing region alone: levels of up to 256-65536 entries
-            // expanded so took the 4096^2 grid from 5.09 to 5.8-6.5 ms.
-            if (a.td_local > 0 && (mode == 2 || (L > 0 && nh == 0 && n >= kLocalPerWave * total_waves)))
-                expand_local<BW>(c, wq, root, beg, adj, allow, vis, qout, qn_out, hs, hout, hn_out, my_found, gmin,
-                                 a.td_local);
-            block_flush_both(c, wq, qout, qn_out, hs, hout, hn_out);
+    my_found = 0;
+    fedges = 0;
+    const int32_t* qin = (L & 1) ? q1 : q0;
+    int32_t* qout = (L & 1) ? q0 : q1;
+    const HubItem* hin = (L & 1) ? h1 : h0;
+    HubItem* hout = (L & 1) ? h0 : h1;
+    int32_t* qn_out = &gs->qn[dir][(L + 1) % 3].v;
+    int32_t* hn_out = &gs->hn[dir][(L + 1) % 3].v;
+    const bool topdown = mode != 1;
+    const bool tr = a.trace && L < kLevelTrace;
+    const int64_t tl0 = tr ? (int64_t)wall_clock64() : 0;
+    if (topdown) {
+        // ---- top-down: tiles of the queue, then hub work items; level 0
+        // expands the root's list in kGiantChunk-edge pieces.
+        int64_t n = 0, nh = 0;
+        int32_t rchunk = kGiantChunk;  // level 0: edges per piece of the root's list
+        if (L == 0 && !multi) {
+            if (c.b == 0 && c.tid == 0)
+                atomicOr(reinterpret_cast<unsigned long long*>(vis + (root >> 6)), 1ull << (root & 63));
+            // the root's list in about one piece per block
+            const int32_t rdeg = beg[root + 1] - beg[root];
+            rchunk = max(256, ((rdeg + c.G - 1) / c.G + 63) & ~63);
+            rchunk = min(rchunk, kGiantChunk);
+            nh = (rdeg + rchunk - 1) / rchunk;
         } else {
-            // ---- bottom-up over the candidate bitmap, one wave per group of
-            // kU consecutive 64-vertex words; only that wave writes those
-            // words in this level.  Each stage issues its kU loads (first
-            // in-neighbours, their visited bits) before any result is used.
-            // Nothing is queued: each word's visited bits before the level go
-            // to `snap`, and a switch back to top-down gathers vis & ~snap.
-            constexpr int kU = 8;       // dense words per group
-            constexpr int kSparse = 8;  // at most this many candidates: a lane's own work
-            constexpr int kS = 4;       // sparse candidates per lane per step
-            int scanned = 0;
-            // candidates whose first probe misses are parked in the wave's LDS
-            // buffer and scanned 64 at a time, one per lane, so their load
-            // chains overlap instead of idling the other 63 lanes
-            int nmiss = 0;  // wave-uniform
-            auto scan_misses = [&]() {
-                wave_sync();
-                int32_t pv = -1;  // found in the previous batch: set behind this batch's loads
-                for (int i0 = 0; i0 < nmiss; i0 += kWave) {
-                    const int i = i0 + c.lane;
-                    int32_t v = 0, p = 0, e = 0;
-                    if (i < nmiss) {
-                        v = wq.buf[i];
-                        p = rbeg[v] + 1;
-                        e = rbeg[v + 1];
+            n = load_agent(&gs->qn[dir][L % 3].v);
+            nh = load_agent(&gs->hn[dir][L % 3].v);
+            if (n < 0 || n > a.V || nh < 0 || nh > a.hub_cap) {  // cannot happen with exact claims
+                if (c.tid == 0) atomicOr(&gs->err.v, 4);
+                n = 0;
+                nh = 0;
+            }
+        }
+        static_assert((kGBlock & (kGBlock - 1)) == 0, "tile size");
+        // entries per tile: a power of two from 64 to kGBlock, the
+        // smallest that needs no more tiles than blocks, so a small
+        // frontier is spread over many blocks
+        int tile = kGBlock;
+        while (tile > kWave && (int64_t)(tile / 2) * c.G >= n) tile /= 2;
+        const int64_t ntiles = (n + tile - 1) / tile;
+        for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
+            if (u < ntiles) {
+                // a tile: queue entries on threads 0..tile-1
+                const int64_t i = u * tile + c.tid;
+                int32_t st = 0, dg = 0;
+                if (c.tid < tile && i < n) {
+                    const int32_t v = load_agent(&qin[i]);
+                    st = beg[v];
+                    dg = beg[v + 1] - st;
+                }
+                int excl, total;
+                block_excl_scan<kW>(dg, excl, total, sm.tmp);
+                sm.off[c.tid] = excl;
+                sm.start[c.tid] = st;
+                if (c.tid == 0) c.edges_td += total;
+                __syncthreads();
+                for (int eb = 0; eb < total; eb += kGBlock * kTK) {
+                    // lane owns edges eb + k * kGBlock + tid; the source of
+                    // edge e is the last j with off[j] <= e (off[0] = 0,
+                    // non-decreasing), found by a fixed-depth search
+                    int e[kTK], pos[kTK];
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) {
+                        e[k] = eb + k * kGBlock + c.tid;
+                        pos[k] = 0;
                     }
-                    if (pv >= 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + (pv >> 6)), 1ull << (pv & 63));
-                    pv = -1;
-                    const bool found = wave_any_edge(i < nmiss && p < e, p, e, v, radj, is_vis, scanned);
-                    if (found) {
-                        pv = v;
-                        my_found += 1;
-                        if (BW) gmin = min(gmin, v);
+#pragma unroll
+                    for (int sp = kGBlock / 2; sp > 0; sp >>= 1) {
+#pragma unroll
+                        for (int k = 0; k < kTK; ++k)
+                            if (sm.off[pos[k] + sp] <= e[k]) pos[k] += sp;
                     }
+                    bool want[kTK];
+                    int32_t w[kTK];
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) {
+                        w[k] = 0;
+                        if (e[k] < total) w[k] = adj[sm.start[pos[k]] + (e[k] - sm.off[pos[k]])];
+                    }
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) want[k] = e[k] < total && w[k] != root;
+                    bool claim[kTK];
+                    int32_t s0[kTK], d[kTK];
+                    claim_with_degree<kTK>(want, w, allow, vis, beg, claim, s0, d);
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) {
+                        my_found += claim[k];
+                    }
+                    push_found_n<kTK>(claim, w, s0, d, wq, qout, qn_out, hs, hout, hn_out, fedges);
+                }
+                __syncthreads();
+            } else {
+                int32_t v, s0;
+                if (L == 0 && !multi) {
+                    v = root;
+                    s0 = beg[root] + (int32_t)(u - ntiles) * rchunk;
+                } else {
+                    v = load_agent(&hin[u - ntiles].v);
+                    s0 = load_agent(&hin[u - ntiles].start);
+                }
+                const int32_t e1 = min(s0 + (L == 0 && !multi ? rchunk : kGiantChunk), beg[v + 1]);
+                if (c.tid == 0) c.edges_td += e1 - s0;
+                for (int32_t eb = s0; eb < e1; eb += kGBlock * kTK) {
+                    bool want[kTK];
+                    int32_t w[kTK];
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) {
+                        const int32_t e = eb + k * kGBlock + c.tid;
+                        w[k] = e < e1 ? adj[e] : 0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) want[k] = eb + k * kGBlock + c.tid < e1 && w[k] != root;
+                    bool claim[kTK];
+                    int32_t ws[kTK], wd[kTK];
+                    claim_with_degree<kTK>(want, w, allow, vis, beg, claim, ws, wd);
+#pragma unroll
+                    for (int k = 0; k < kTK; ++k) {
+                        my_found += claim[k];
+                    }
+                    push_found_n<kTK>(claim, w, ws, wd, wq, qout, qn_out, hs, hout, hn_out, fedges);
+                }
+            }
+        }
+        // in place: in top-down to exhaustion, and in any later top-down
+        // level whose input gives every wave at least kLocalPerWave
+        // vertices.  Where the frontier is small (the start of a
+        // traversal on a high-diameter graph) a few waves would expand a
+        // growing region alone: levels of up to 256-65536 entries
+        // expanded so took the 4096^2 grid from 5.09 to 5.8-6.5 ms.
+        if (a.td_local > 0 && (mode == 2 || (L > 0 && nh == 0 && n >= kLocalPerWave * total_waves)))
+            expand_local<BW>(c, wq, root, beg, adj, allow, vis, qout, qn_out, hs, hout, hn_out, my_found,
+                             a.td_local);
+        block_flush_both(c, wq, qout, qn_out, hs, hout, hn_out);
+    } else {
+        // ---- bottom-up over the candidate bitmap, one wave per group of
+        // kU consecutive 64-vertex words; only that wave writes those
+        // words in this level.  Each stage issues its kU loads (first
+        // in-neighbours, their visited bits) before any result is used.
+        // Nothing is queued: each word's visited bits before the level go
+        // to `snap`, and a switch back to top-down gathers vis & ~snap.
+        constexpr int kU = 8;       // dense words per group
+        constexpr int kSparse = 8;  // at most this many candidates: a lane's own work
+        constexpr int kS = 4;       // sparse candidates per lane per step
+        int scanned = 0;
+        // candidates whose first probe misses are parked in the wave's LDS
+        // buffer and scanned 64 at a time, one per lane, so their load
+        // chains overlap instead of idling the other 63 lanes
+        int nmiss = 0;  // wave-uniform
+        auto scan_misses = [&]() {
+            wave_sync();
+            int32_t pv = -1;  // found in the previous batch: set behind this batch's loads
+            for (int i0 = 0; i0 < nmiss; i0 += kWave) {
+                const int i = i0 + c.lane;
+                int32_t v = 0, p = 0, e = 0;
+                if (i < nmiss) {
+                    v = wq.buf[i];
+                    p = rbeg[v] + 1;
+                    e = rbeg[v + 1];
                 }
                 if (pv >= 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + (pv >> 6)), 1ull << (pv & 63));
-                nmiss = 0;
-                wave_sync();
-            };
-            // misses with more list to scan go to the wave's buffer (all lanes call)
-            auto park_misses = [&](bool miss, int32_t v) {
-                const uint64_t mm = __ballot(miss);
-                if (!mm) return;
-                if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
-                if (miss) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
-                nmiss += __popcll(mm);
-            };
-            // A wave step covers kWave / kBUrun runs of kBUrun consecutive words
-            // spread over the whole id range (run r of the step is run
-            // cb + r * total_waves + gwave), so that the id ranges where the
-            // work concentrates are shared by all blocks.
-            constexpr int kBUrun = 8;
-            const int64_t nruns = (a.nwords + kBUrun - 1) / kBUrun;
-            // All waves sweep the id range in the same order (band by band:
-            // vertices found early help later probes of the level hit), and
-            // within a band wave g takes run (g * bu_mult) mod total_waves, a
-            // permutation: with plain indices a block's runs would share
-            // fixed bits of their vertex ids, and in graphs whose degrees
-            // follow id bits (R-MAT) the blocks' work would differ severalfold.
-            // (bu_mult < total_waves <= 65536: the product fits 32 bits)
-            const int64_t pg = (int64_t)(((uint32_t)gwave * (uint32_t)a.bu_mult) % (uint32_t)total_waves);
-            for (int64_t cb = 0; cb < nruns; cb += total_waves * (kWave / kBUrun)) {
-                // lane j loads one word; then only words with candidates are
-                // worked on, kU of them at a time
-                const int64_t run = cb + (int64_t)(c.lane / kBUrun) * total_waves + pg;
-                const int64_t wl = run * kBUrun + (c.lane % kBUrun);
-                uint64_t lvw = 0, lcand = 0;
-                if (wl < a.nwords) {
-                    lvw = vis[wl];
-                    lcand = allow[wl] & ~lvw;
+                pv = -1;
+                const bool found = wave_any_edge(i < nmiss && p < e, p, e, v, radj, is_vis, scanned);
+                if (found) {
+                    pv = v;
+                    my_found += 1;
                 }
-                // words with more than kSparse candidates: the wave works on
-                // kU of them at a time, one lane per vertex
-                const bool dense = __popcll(lcand) > kSparse;
-                uint64_t busy = __ballot(dense);
-                // Found bits of group member u wait in lane u and go out as
-                // one atomic instruction behind the next group's loads: a
-                // load's wait also waits for the atomics issued before it.
-                int64_t pw = 0;
-                uint64_t pm = 0;
-                while (busy) {
-                    int js[kU];
+            }
+            if (pv >= 0) atomicOr(reinterpret_cast<unsigned long long*>(vis + (pv >> 6)), 1ull << (pv & 63));
+            nmiss = 0;
+            wave_sync();
+        };
+        // misses with more list to scan go to the wave's buffer (all lanes call)
+        auto park_misses = [&](bool miss, int32_t v) {
+            const uint64_t mm = __ballot(miss);
+            if (!mm) return;
+            if (nmiss + __popcll(mm) > kWaveBuf) scan_misses();
+            if (miss) wq.buf[nmiss + __popcll(mm & lanemask_lt())] = v;
+            nmiss += __popcll(mm);
+        };
+        // A wave step covers kWave / kBUrun runs of kBUrun consecutive words
+        // spread over the whole id range (run r of the step is run
+        // cb + r * total_waves + gwave), so that the id ranges where the
+        // work concentrates are shared by all blocks.
+        constexpr int kBUrun = 8;
+        const int64_t nruns = (a.nwords + kBUrun - 1) / kBUrun;
+        // All waves sweep the id range in the same order (band by band:
+        // vertices found early help later probes of the level hit), and
+        // within a band wave g takes run (g * bu_mult) mod total_waves, a
+        // permutation: with plain indices a block's runs would share
+        // fixed bits of their vertex ids, and in graphs whose degrees
+        // follow id bits (R-MAT) the blocks' work would differ severalfold.
+        // (bu_mult < total_waves <= 65536: the product fits 32 bits)
+        const int64_t pg = (int64_t)(((uint32_t)gwave * (uint32_t)a.bu_mult) % (uint32_t)total_waves);
+        for (int64_t cb = 0; cb < nruns; cb += total_waves * (kWave / kBUrun)) {
+            // lane j loads one word; then only words with candidates are
+            // worked on, kU of them at a time
+            const int64_t run = cb + (int64_t)(c.lane / kBUrun) * total_waves + pg;
+            const int64_t wl = run * kBUrun + (c.lane % kBUrun);
+            uint64_t lvw = 0, lcand = 0;
+            if (wl < a.nwords) {
+                lvw = vis[wl];
+                lcand = allow[wl] & ~lvw;
+            }
+            // words with more than kSparse candidates: the wave works on
+            // kU of them at a time, one lane per vertex
+            const bool dense = __popcll(lcand) > kSparse;
+            uint64_t busy = __ballot(dense);
+            // Found bits of group member u wait in lane u and go out as
+            // one atomic instruction behind the next group's loads: a
+            // load's wait also waits for the atomics issued before it.
+            int64_t pw = 0;
+            uint64_t pm = 0;
+            while (busy) {
+                int js[kU];
 #pragma unroll
-                    for (int u = 0; u < kU; ++u) {
-                        js[u] = busy ? __ffsll((unsigned long long)busy) - 1 : -1;
-                        busy &= busy ? busy - 1 : 0;
-                    }
-                    int32_t f[kU];
-                    int64_t ws[kU];  // word of group member u (wave-uniform)
-#pragma unroll
-                    for (int u = 0; u < kU; ++u) {
-                        f[u] = -1;
-                        ws[u] = js[u] >= 0 ? lane_get(wl, js[u]) : 0;
-                        if (js[u] >= 0 && ((lane_get(lcand, js[u]) >> c.lane) & 1ull))
-                            f[u] = rfirst[ws[u] * 64 + c.lane];
-                    }
-                    if (pm) atomicOr(reinterpret_cast<unsigned long long*>(vis + pw), (unsigned long long)pm);
-                    pm = 0;
-                    uint32_t hits = 0;
-#pragma unroll
-                    for (int u = 0; u < kU; ++u) hits |= (f[u] >= 0 && bit_of(vis, f[u])) ? 1u << u : 0u;
-#pragma unroll
-                    for (int u = 0; u < kU; ++u) {
-                        if (js[u] < 0) continue;
-                        const uint64_t cm = lane_get(lcand, js[u]);
-                        const int64_t w = ws[u];
-                        const int32_t v = (int32_t)(w * 64 + c.lane);
-                        const bool is_c = (cm >> c.lane) & 1ull;
-                        const bool hit = is_c && ((hits >> u) & 1u);
-                        scanned += is_c;
-                        park_misses(is_c && !hit && f[u] >= 0, v);
-                        // found bits are set within the level: later probes
-                        // already see them
-                        const uint64_t fm = __ballot(hit);
-                        if (c.lane == u) {
-                            pw = w;
-                            pm = fm;
-                        }
-                        my_found += hit;
-                        if (BW && hit) gmin = min(gmin, v);
-                    }
+                for (int u = 0; u < kU; ++u) {
+                    js[u] = busy ? __ffsll((unsigned long long)busy) - 1 : -1;
+                    busy &= busy ? busy - 1 : 0;
                 }
-                if (tr && c.tid == 0 && cb == 0)
-                    atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][3]),
-                              (unsigned long long)((int64_t)wall_clock64() - tl0));
-                // sparse words: each lane works through its own word's
-                // candidates, kS at a time, and sets its found bits at once
-                uint64_t mine = dense ? 0ull : lcand, fm_lane = 0;
-                while (__ballot(mine != 0)) {
-                    int32_t v[kS], f[kS];
+                int32_t f[kU];
+                int64_t ws[kU];  // word of group member u (wave-uniform)
 #pragma unroll
-                    for (int k = 0; k < kS; ++k) {
-                        v[k] = -1;
-                        if (mine) {
-                            v[k] = (int32_t)(wl * 64 + __ffsll((unsigned long long)mine) - 1);
-                            mine &= mine - 1;
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < kS; ++k) f[k] = v[k] >= 0 ? rfirst[v[k]] : -1;
-                    bool hit[kS];
-#pragma unroll
-                    for (int k = 0; k < kS; ++k) hit[k] = f[k] >= 0 && bit_of(vis, f[k]);
-#pragma unroll
-                    for (int k = 0; k < kS; ++k) {
-                        scanned += v[k] >= 0;
-                        park_misses(v[k] >= 0 && !hit[k] && f[k] >= 0, v[k]);
-                        if (hit[k]) {
-                            fm_lane |= 1ull << (v[k] & 63);
-                            my_found += 1;
-                            if (BW) gmin = min(gmin, v[k]);
-                        }
-                    }
+                for (int u = 0; u < kU; ++u) {
+                    f[u] = -1;
+                    ws[u] = js[u] >= 0 ? lane_get(wl, js[u]) : 0;
+                    if (js[u] >= 0 && ((lane_get(lcand, js[u]) >> c.lane) & 1ull))
+                        f[u] = rfirst[ws[u] * 64 + c.lane];
                 }
                 if (pm) atomicOr(reinterpret_cast<unsigned long long*>(vis + pw), (unsigned long long)pm);
-                if (fm_lane) atomicOr(reinterpret_cast<unsigned long long*>(vis + wl), (unsigned long long)fm_lane);
-                if (wl < a.nwords) st_agent(&a.snap[wl], lvw);  // the gather's "visited before this level"
-            }
-            if (tr && c.tid == 0)  // wave 0 of each block
-                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][1]),
-                          (unsigned long long)((int64_t)wall_clock64() - tl0));
-            scan_misses();
-            if (tr && c.tid == 0)
-                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][2]),
-                          (unsigned long long)((int64_t)wall_clock64() - tl0));
-            fedges = scanned;  // bottom-up pushes nothing: the low half of the level sum counts the scan
-        }
-        // ---- level totals: every block computes the same sums
-        // one block reduction: finds (high 32 bits) + pushed out-degrees (low
-        // 32 bits, at most E < 2^31), and backward the smallest id reached
-        int64_t pk = wave_sum((my_found << 32) + fedges);
-        int32_t bm = BW ? wave_min(gmin) : INT32_MAX;
-        if (c.lane == 0) {
-            sm.red[c.wave] = pk;
-            sm.tmp[c.wave] = bm;
-        }
-        __syncthreads();
-        pk = lane_get(row_incl(sm.red[c.lane & 15], (int64_t)0, [](int64_t x, int64_t y) { return x + y; }), 15);
-        if (BW) bm = lane_get(row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); }), 15);
-        __syncthreads();
-        const int64_t blk = (int64_t)((uint64_t)pk >> 32);
-        if (topdown) {
-            c.found_td += blk;
-        } else {
-            const int64_t bscan = pk & 0xffffffffll;
-            c.found_bu += blk;
-            c.edges_bu += bscan;
-            if (tr && c.tid == 0) {
-                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][0]), (unsigned long long)bscan);
-                if (!BW && c.b < 1024 && gs->bu1_scan[c.b] == 0) {
-                    gs->bu1_scan[c.b] = bscan + 1;
-                    gs->bu1_work[c.b] = (int64_t)wall_clock64() - tl0;
+                pm = 0;
+                uint32_t hits = 0;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) hits |= (f[u] >= 0 && bit_of(vis, f[u])) ? 1u << u : 0u;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    if (js[u] < 0) continue;
+                    const uint64_t cm = lane_get(lcand, js[u]);
+                    const int64_t w = ws[u];
+                    const int32_t v = (int32_t)(w * 64 + c.lane);
+                    const bool is_c = (cm >> c.lane) & 1ull;
+                    const bool hit = is_c && ((hits >> u) & 1u);
+                    scanned += is_c;
+                    park_misses(is_c && !hit && f[u] >= 0, v);
+                    // found bits are set within the level: later probes
+                    // already see them
+                    const uint64_t fm = __ballot(hit);
+                    if (c.lane == u) {
+                        pw = w;
+                        pm = fm;
+                    }
+                    my_found += hit;
                 }
             }
+            if (tr && c.tid == 0 && cb == 0)
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][3]),
+                          (unsigned long long)((int64_t)wall_clock64() - tl0));
+            // sparse words: each lane works through its own word's
+            // candidates, kS at a time, and sets its found bits at once
+            uint64_t mine = dense ? 0ull : lcand, fm_lane = 0;
+            while (__ballot(mine != 0)) {
+                int32_t v[kS], f[kS];
+#pragma unroll
+                for (int k = 0; k < kS; ++k) {
+                    v[k] = -1;
+                    if (mine) {
+                        v[k] = (int32_t)(wl * 64 + __ffsll((unsigned long long)mine) - 1);
+                        mine &= mine - 1;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kS; ++k) f[k] = v[k] >= 0 ? rfirst[v[k]] : -1;
+                bool hit[kS];
+#pragma unroll
+                for (int k = 0; k < kS; ++k) hit[k] = f[k] >= 0 && bit_of(vis, f[k]);
+#pragma unroll
+                for (int k = 0; k < kS; ++k) {
+                    scanned += v[k] >= 0;
+                    park_misses(v[k] >= 0 && !hit[k] && f[k] >= 0, v[k]);
+                    if (hit[k]) {
+                        fm_lane |= 1ull << (v[k] & 63);
+                        my_found += 1;
+                    }
+                }
+            }
+            if (pm) atomicOr(reinterpret_cast<unsigned long long*>(vis + pw), (unsigned long long)pm);
+            if (fm_lane) atomicOr(reinterpret_cast<unsigned long long*>(vis + wl), (unsigned long long)fm_lane);
+            if (wl < a.nwords) st_agent(&snap[wl], lvw);  // the gather's "visited before this level"
         }
-        const int64_t key = BW ? (int64_t)INT32_MAX - bm : 0;  // BW: INT32_MAX - (smallest id reached)
-        if (tr && c.tid == 0) {
-            const unsigned long long wt = (unsigned long long)((int64_t)wall_clock64() - tl0);
-            atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmax[dir][L]), wt);
-            atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmin[dir][L]), ~wt);
+        if (tr && c.tid == 0)  // wave 0 of each block
+            atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][1]),
+                      (unsigned long long)((int64_t)wall_clock64() - tl0));
+        scan_misses();
+        if (tr && c.tid == 0)
+            atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][L][2]),
+                      (unsigned long long)((int64_t)wall_clock64() - tl0));
+        fedges = scanned;  // bottom-up pushes nothing: the low half of the level sum counts the scan
+    }
+}
+
+// The gather of a direction that switches from bottom-up back to top-down:
+// the last level's finds (vis & ~snap) become the next level's queue and hub
+// items.  Each wave loads 64 words at once, then expands the non-empty ones.
+template <bool BW>
+__device__ __forceinline__ void gather_finds(Ctx& c, const Dir& d) {
+    const GiantArgs& a = c.a;
+    Smem& sm = c.sm;
+    GiantState* gs = a.gs;
+    const int dir = BW ? 1 : 0;
+    const int32_t* beg = BW ? a.bwb : a.fwb;
+    const uint64_t* vis = BW ? a.bvis : a.fvis;
+    const uint64_t* snap = BW ? a.snapb : a.snap;
+    int32_t* q_next = (d.L & 1) ? (BW ? a.qb1 : a.q1) : (BW ? a.qb0 : a.q0);
+    HubItem* h_next = (d.L & 1) ? (BW ? a.hb1 : a.h1) : (BW ? a.hb0 : a.h0);
+    int32_t* qn_next = &gs->qn[dir][d.L % 3].v;
+    int32_t* hn_next = &gs->hn[dir][d.L % 3].v;
+    WaveQueue wq{sm.wbuf[c.wave], 0};
+    HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
+    const int64_t total_waves = (int64_t)c.G * kW;
+    const int64_t gwave = (int64_t)c.b * kW + c.wave;
+    int64_t unused = 0;
+    for (int64_t wb = gwave * kWave; wb < a.nwords; wb += total_waves * kWave) {
+        const int64_t w = wb + c.lane;
+        uint64_t nw = 0;
+        if (w < a.nwords) nw = vis[w] & ~snap[w];
+        uint64_t busy = __ballot(nw != 0);
+        while (busy) {
+            const int l = __ffsll((unsigned long long)busy) - 1;
+            busy &= busy - 1;
+            const uint64_t m = lane_get(nw, l);
+            const int32_t v = (int32_t)((wb + l) * 64 + c.lane);
+            push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next, unused);
         }
+    }
+    block_flush_both(c, wq, q_next, qn_next, hs, h_next, hn_next);
+}
+
+// The giant's traversals: forward and backward in the same levels, one grid
+// barrier per level for both (fw_bfs_fq_queue, src/fw_bw.h:396-698, and
+// bw_bfs_fq_queue, :944-1248, which the reference runs one after the
+// other).  Each direction switches top-down / bottom-up / top-down to
+// exhaustion on its own, by the reference's rules on alpha, beta and gamma
+// (src/fw_bw.h:576-624) with set_size the active count for both; a
+// direction that has finished idles while the other goes on.  Returns false
+// after a failed barrier.  With a.n_roots > 0 every listed root seeds both
+// directions (*seeded_out of them).
+__device__ __forceinline__ bool traverse_joint(Ctx& c, int32_t root, int64_t set_size, Dir& f, Dir& b,
+                                               int64_t* seeded_out) {
+    const GiantArgs& a = c.a;
+    Smem& sm = c.sm;
+    GiantState* gs = a.gs;
+    f = Dir{0, 0, 0, true, false};
+    b = Dir{0, 0, 0, true, false};
+    *seeded_out = 1;
+    if (a.n_roots > 0) {
+        // level-0 input of both directions: every graph's root, marked
+        // visited, queued or split into hub items like any claimed vertex
+        int64_t seeded = 0;
+        for (int dir = 0; dir < 2; ++dir) {
+            WaveQueue wq{sm.wbuf[c.wave], 0};
+            HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
+            const int32_t* beg = dir ? a.bwb : a.fwb;
+            uint64_t* vis = dir ? a.bvis : a.fvis;
+            int32_t* q0 = dir ? a.qb0 : a.q0;
+            HubItem* h0 = dir ? a.hb0 : a.h0;
+            int64_t unused = 0;
+            for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < a.n_roots; ib += (int64_t)c.G * kGBlock) {
+                const int64_t i = ib + c.lane;
+                const int32_t r = i < a.n_roots ? a.roots[i] : -1;
+                const bool ok = r >= 0 && r < a.V;
+                if (ok) atomicOr(reinterpret_cast<unsigned long long*>(vis + (r >> 6)), 1ull << (r & 63));
+                if (dir == 0) seeded += ok;
+                push_next(ok, r, beg, wq, q0, &gs->qn[dir][0].v, hs, h0, &gs->hn[dir][0].v, unused);
+            }
+            block_flush_both(c, wq, q0, &gs->qn[dir][0].v, hs, h0, &gs->hn[dir][0].v);
+        }
+        Red r;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{block_sum(seeded, sm), 0, 0, 0}, &r, 1)) return false;
+        *seeded_out = r.s0;
+    }
+    while (f.live || b.live) {
+        if (f.gather || b.gather) {
+            // the visited bits were made exact after the level's barrier
+            if (f.gather) gather_finds<false>(c, f);
+            if (b.gather) gather_finds<true>(c, b);
+            f.gather = b.gather = false;
+            // the top-down level reads the queues coherently and visited bits
+            // only as hints: no L2 invalidation
+            if (!gsync<false, false>(gs, c.G, sm, a.dbg)) return false;
+        }
+        const bool tdf = f.mode != 1, tdb = b.mode != 1;
+        const int64_t edges0 = c.edges_td + c.edges_bu;  // thread 0's are the block's
+        const bool tr = a.trace != 0;
+        const int64_t tl0 = tr ? (int64_t)wall_clock64() : 0;
+        int64_t ff = 0, fe = 0, bf = 0, be = 0;  // per thread: found, pushed work / scanned entries
+        if (f.live) level_work<false>(c, f, root, ff, fe);
+        if (b.live) level_work<true>(c, b, root, bf, be);
+        // ---- level totals: every block computes the same sums.  One sum per
+        // direction carries the level's finds (high 32 bits) and the next
+        // level's exact work or the scan (low 32 bits: at most E < 2^31, so no
+        // carry), both reduced in one block round
+        int64_t pf = wave_sum((ff << 32) + fe), pb = wave_sum((bf << 32) + be);
+        if (c.lane == 0) {
+            sm.red[c.wave] = pf;
+            sm.red2[c.wave] = pb;
+        }
+        __syncthreads();
+        pf = lane_get(row_incl(sm.red[c.lane & 15], (int64_t)0, [](int64_t x, int64_t y) { return x + y; }), 15);
+        pb = lane_get(row_incl(sm.red2[c.lane & 15], (int64_t)0, [](int64_t x, int64_t y) { return x + y; }), 15);
+        __syncthreads();
+        auto block_counts = [&](const Dir& d, int64_t pk, bool td, int dir) {
+            const int64_t blk = (int64_t)((uint64_t)pk >> 32);
+            if (td) {
+                c.found_td += blk;
+            } else {
+                const int64_t bscan = pk & 0xffffffffll;
+                c.found_bu += blk;
+                c.edges_bu += bscan;
+                if (tr && d.L < kLevelTrace && c.tid == 0) {
+                    atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_cp[dir][d.L][0]), (unsigned long long)bscan);
+                    if (dir == 0 && c.b < 1024 && gs->bu1_scan[c.b] == 0) {
+                        gs->bu1_scan[c.b] = bscan + 1;
+                        gs->bu1_work[c.b] = (int64_t)wall_clock64() - tl0;
+                    }
+                }
+            }
+            if (tr && d.L < kLevelTrace && c.tid == 0) {
+                const unsigned long long wt = (unsigned long long)((int64_t)wall_clock64() - tl0);
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmax[dir][d.L]), wt);
+                atomicMax(reinterpret_cast<unsigned long long*>(&gs->lv_wmin[dir][d.L]), ~wt);
+            }
+            // counters of level L+2: last read at the start of level L-1,
+            // first written in level L+1 (reset here, where the store overlaps
+            // the barrier's own atomics)
+            if (c.b == 0 && c.tid == 0) {
+                __hip_atomic_store(&gs->qn[dir][(d.L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&gs->hn[dir][(d.L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        };
+        if (f.live) block_counts(f, pf, tdf, 0);
+        if (b.live) block_counts(b, pb, tdb, 1);
         // Top-down levels read what other blocks wrote in the level before
         // through coherent loads (queue entries, hub items, counts), a
         // candidate bitmap that is fixed during the traversal, and visited
         // bits only as a hint for the claim atomics: their barriers skip the
         // L2 invalidation (the acquire below catches up where needed).
-        // One sum carries the level's finds (high 32 bits) and the next level's
-        // exact work (low 32 bits: at most E < 2^31, so no carry); the
-        // adjacency count only feeds the level trace, and the smallest id only
-        // matters backward.
-        // counters of level L+2: last read at the start of level L-1, first
-        // written in level L+1 (reset here, where the store overlaps the
-        // barrier's own atomics instead of delaying this level's first load)
-        if (c.b == 0 && c.tid == 0) {
-            __hip_atomic_store(&gs->qn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gs->hn[dir][(L + 2) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
         Red r;
-        if (!gsync<true, false>(gs, c.G, sm, a.dbg,
-                                Red{pk, a.trace ? c.edges_td + c.edges_bu - edges0 : 0, 0, key}, &r,
-                                1 | (a.trace ? 2 : 0) | (BW ? 8 : 0)))
-            return -1;
-        const int64_t found = (int64_t)((uint64_t)r.s0 >> 32);
-        if (c.b == 0 && c.tid == 0 && L < kLevelTrace) {
-            gs->lv_ts[dir][L] = wall_clock64();
-            gs->lv_found[dir][L] = found;
-            gs->lv_edges[dir][L] = r.s1;
-            gs->lv_mode[dir][L] = topdown ? 0 : 1;
-        }
-        if (BW) *gmin_out = (int32_t)((int64_t)INT32_MAX - r.key);
-        const int64_t next_edges = r.s0 & 0xffffffffll;  // top-down levels: exact work of the next level
-        visited += found;
-        ++L;
-        if (found == 0) {
-            block_acquire();  // what follows reads the visited bits exactly
-            break;
-        }
-        if (topdown && a.td_local > 0 &&  // the waves may have expanded everything they found
-            load_agent(&gs->qn[dir][L % 3].v) == 0 && load_agent(&gs->hn[dir][L % 3].v) == 0) {
-            block_acquire();
-            break;
-        }
-        if (L > (int64_t)a.V + 2) {
-            if (c.tid == 0) atomicOr(&gs->err.v, 2);
-            return -1;
-        }
-        if (mode == 0) {
-            const double ef = a.switch_rule ? (double)next_edges : (double)found * a.avg_degree;
-            const double er = (double)(set_size - visited) * a.avg_degree;
-            if (er / a.alpha < ef) mode = 1;
-        }
-        // bottom-up levels and the gather read visited bits exactly
-        if (!topdown || mode == 1) block_acquire();
-        if (!topdown && ((set_size * 1.0 / a.beta) > found || L - 1 > a.gamma)) {  // this level was bottom-up
-            // gather this level's finds (vis & ~snap) as the next top-down
-            // input: each wave loads 64 words at once, then expands the
-            // non-empty ones
-            int32_t* q_next = (L & 1) ? a.q1 : a.q0;
-            HubItem* h_next = (L & 1) ? a.h1 : a.h0;
-            int32_t* qn_next = &gs->qn[dir][L % 3].v;
-            int32_t* hn_next = &gs->hn[dir][L % 3].v;
-            for (int64_t wb = gwave * kWave; wb < a.nwords; wb += total_waves * kWave) {
-                const int64_t w = wb + c.lane;
-                uint64_t nw = 0;
-                if (w < a.nwords) nw = vis[w] & ~a.snap[w];
-                uint64_t busy = __ballot(nw != 0);
-                while (busy) {
-                    const int l = __ffsll((unsigned long long)busy) - 1;
-                    busy &= busy - 1;
-                    const uint64_t m = lane_get(nw, l);
-                    const int32_t v = (int32_t)((wb + l) * 64 + c.lane);
-                    push_next((m >> c.lane) & 1ull, v, beg, wq, q_next, qn_next, hs, h_next, hn_next, fedges);
-                }
+        if (!gsync<true, false>(gs, c.G, sm, a.dbg, Red{pf, pb, tr ? c.edges_td + c.edges_bu - edges0 : 0, 0}, &r,
+                                3 | (tr ? 4 : 0)))
+            return false;
+        // per direction: the level's totals, the end test and the switches
+        bool acquire = false, failed = false;
+        auto decide = [&](Dir& d, int64_t pk, bool td, int dir) {
+            const int64_t found = (int64_t)((uint64_t)pk >> 32);
+            const int64_t next_edges = pk & 0xffffffffll;  // top-down levels: exact work of the next level
+            if (c.b == 0 && c.tid == 0 && d.L < kLevelTrace) {
+                gs->lv_ts[dir][d.L] = wall_clock64();
+                gs->lv_found[dir][d.L] = found;
+                gs->lv_edges[dir][d.L] = r.s2;
+                gs->lv_mode[dir][d.L] = td ? 0 : 1;
             }
-            block_flush_both(c, wq, q_next, qn_next, hs, h_next, hn_next);
-            mode = 2;
-            // the top-down level reads the queue coherently and visited bits
-            // only as hints: no L2 invalidation (the one above covered the
-            // gather's exact reads)
-            if (!gsync<false, false>(gs, c.G, sm, a.dbg)) return -1;
-        }
+            d.visited += found;
+            ++d.L;
+            if (found == 0 ||
+                (td && a.td_local > 0 &&  // the waves may have expanded everything they found
+                 load_agent(&gs->qn[dir][d.L % 3].v) == 0 && load_agent(&gs->hn[dir][d.L % 3].v) == 0)) {
+                d.live = false;
+                acquire = true;  // what follows reads the visited bits exactly
+                if (dir == 0 && c.b == 0 && c.tid == 0) gs->ts[2] = wall_clock64();  // the forward one's end
+                return;
+            }
+            if (d.L > (int64_t)a.V + 2) {
+                if (c.tid == 0) atomicOr(&gs->err.v, 2);
+                failed = true;
+                return;
+            }
+            if (d.mode == 0) {
+                const double ef = a.switch_rule ? (double)next_edges : (double)found * a.avg_degree;
+                const double er = (double)(set_size - d.visited) * a.avg_degree;
+                if (er / a.alpha < ef) d.mode = 1;
+            }
+            // bottom-up levels and the gather read visited bits exactly
+            if (!td || d.mode == 1) acquire = true;
+            if (!td && ((set_size * 1.0 / a.beta) > found || d.L - 1 > a.gamma)) {  // this level was bottom-up
+                d.mode = 2;
+                d.gather = true;
+            }
+        };
+        if (f.live) decide(f, r.s0, tdf, 0);
+        if (b.live) decide(b, r.s1, tdb, 1);
+        if (failed) return false;
+        if (acquire) block_acquire();
     }
-    *levels_out = L;
-    return visited;
+    return true;
 }
 
 // ---- phases after the giant (run when GiantArgs::rest != 0) --------------
@@ -1530,28 +1595,47 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     const int32_t root = (a.n_roots == 0 && bestp > 0 && besti < a.V) ? besti : -1;
     if (stamp) gs->ts[1] = wall_clock64();
 
-    int64_t fw_count = 0, giant = 0, fw_levels = 0, bw_levels = 0;
+    int64_t fw_count = 0, bw_reach = 0, fw_levels = 0, bw_levels = 0;
     int32_t gmin = root;
     const bool multi = a.n_roots > 0;
     const bool traversed = multi || root >= 0;
     if (traversed) {
         int64_t seeded = 0;
-        const int64_t fwc = traverse<false>(c, root, fq0, &fw_levels, &gmin, &seeded);
-        if (fwc < 0) return;
-        fw_count = fwc + seeded;
-        if (stamp) gs->ts[2] = wall_clock64();
-        const int64_t bwc = traverse<true>(c, root, fw_count, &bw_levels, &gmin, &seeded);
-        if (bwc < 0) return;
-        giant = bwc + seeded;
+        Dir f, b;
+        if (!traverse_joint(c, root, fq0, f, b, &seeded)) return;
+        fw_count = f.visited + seeded;
+        bw_reach = b.visited + seeded;
+        fw_levels = f.L;
+        bw_levels = b.L;
+        if (!multi) {
+            // the giant's smallest id (its label): no larger than the pivot,
+            // which the giant contains, so only words up to the pivot's count
+            int32_t mn = INT32_MAX;
+            for (int64_t w = (int64_t)c.b * kGBlock + c.tid; w <= (root >> 6); w += (int64_t)c.G * kGBlock) {
+                const uint64_t gw = a.fvis[w] & a.bvis[w];
+                if (gw) mn = min(mn, (int32_t)(w * 64 + __ffsll((unsigned long long)gw) - 1));
+            }
+            mn = wave_min(mn);
+            if (c.lane == 0) sm.tmp[c.wave] = mn;
+            __syncthreads();
+            mn = lane_get(row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); }), 15);
+            __syncthreads();
+            Red r;
+            if (!gsync<true>(gs, c.G, sm, a.dbg, Red{0, 0, 0, (int64_t)INT32_MAX - mn}, &r, 8)) return;
+            gmin = (int32_t)((int64_t)INT32_MAX - r.key);
+        }
     }
-    if (stamp) gs->ts[3] = wall_clock64();
+    if (stamp) {
+        if (!traversed) gs->ts[2] = wall_clock64();
+        gs->ts[3] = wall_clock64();  // both directions done
+    }
 
     // ---- post: giant labels, act minus giant, every label written once;
     // each block owns a contiguous word range so the rebuilt active queue
     // comes out ascending without atomics.
     const int64_t per = (a.nwords + c.G - 1) / c.G;
     const int64_t wb = min((int64_t)c.b * per, a.nwords), we = min(wb + per, a.nwords);
-    int64_t cnt = 0;
+    int64_t cnt = 0, gcnt = 0;
     // one word per lane: the wave loads 64 words at once, then writes their
     // labels word by word (coalesced 256 B stores, no loads in the loop)
     for (int64_t base = wb + (int64_t)c.wave * kWave; base < we; base += (int64_t)kW * kWave) {
@@ -1559,10 +1643,15 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         uint64_t a0 = 0, g = 0, sw = 0;
         if (w < we) {
             a0 = a.act[w];
-            g = traversed ? a.bvis[w] : 0ull;
+            if (traversed) {  // the giant: reached both ways; bvis keeps only it
+                const uint64_t bw = a.bvis[w];
+                g = a.fvis[w] & bw;
+                if (g != bw) st_agent(&a.bvis[w], g);
+            }
             sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
             if (g) st_agent(&a.act[w], a0 & ~g);
             cnt += __popcll(a0 & ~g);
+            gcnt += __popcll(g);
         }
         const int nw = (int)min<int64_t>(kWave, we - base);
         for (int j = 0; j < nw; ++j) {
@@ -1574,10 +1663,11 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
             }
         }
     }
-    const int64_t cnt_blk = block_sum(cnt, sm);
+    const int64_t cnt_blk = block_sum(cnt, sm), giant_blk = block_sum(gcnt, sm);
     if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[0], (unsigned long long)wall_clock64());
     if (c.tid == 0) {
         st_agent(&a.part[(int64_t)kPost * c.G + c.b], (int64_t)(cnt_blk));
+        st_agent(&a.part[(int64_t)kGiantN * c.G + c.b], (int64_t)(giant_blk));
         st_agent(&a.part[(int64_t)kEdgeTD * c.G + c.b], (int64_t)(c.edges_td));
         st_agent(&a.part[(int64_t)kEdgeBU * c.G + c.b], (int64_t)(c.edges_bu));
         st_agent(&a.part[(int64_t)kFoundTD * c.G + c.b], (int64_t)(c.found_td));
@@ -1615,11 +1705,13 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     if (c.b == 0) {
         const int64_t w0 = sum_slot(a.part, kEdgeTD, c.G, sm), w1 = sum_slot(a.part, kEdgeBU, c.G, sm);
         const int64_t w2 = sum_slot(a.part, kFoundTD, c.G, sm), w3 = sum_slot(a.part, kFoundBU, c.G, sm);
+        const int64_t giant = sum_slot(a.part, kGiantN, c.G, sm);
         if (c.tid == 0) {
             gs->root = root;
             gs->fq0 = fq0;
             gs->fw_count = fw_count;
             gs->giant = giant;
+            gs->bw_reach = bw_reach;
             gs->gmin = gmin;
             gs->fq_n = tot;
             gs->fw_levels = fw_levels;

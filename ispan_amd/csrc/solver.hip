@@ -72,7 +72,9 @@ struct ispan_ctx {
     bool level_trace = false;  // ISPAN_LEVEL_TRACE=1: per-level lines of the giant phase on stderr
     double clock_khz = 0.0;
     uint64_t* snap = nullptr;
-    HubItem* gh[2] = {nullptr, nullptr};
+    HubItem* gh[2] = {nullptr, nullptr};   // forward hub work items
+    HubItem* ghb[2] = {nullptr, nullptr};  // backward
+    uint64_t* snapb = nullptr;             // backward bottom-up snapshot
     int64_t* gpart = nullptr;
     int32_t* gdbg = nullptr;
     GiantState* gstate = nullptr;
@@ -163,6 +165,9 @@ void free_solve_buffers(ispan_ctx* ctx) {
     dfree(ctx->snap);
     dfree(ctx->gh[0]);
     dfree(ctx->gh[1]);
+    dfree(ctx->ghb[0]);
+    dfree(ctx->ghb[1]);
+    dfree(ctx->snapb);
     dfree(ctx->gpart);
     dfree(ctx->gdbg);
     if (ctx->sort_tmp) (void)hipFree(ctx->sort_tmp);
@@ -246,7 +251,11 @@ int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
     CK(hipMalloc(&ctx->qbuf, nv * 4));
     CK(hipMalloc(&ctx->part, (size_t)ctx->grid * sizeof(PivotPart)));
     CK(hipMalloc(&ctx->snap, nw * 8));
-    for (int i = 0; i < 2; ++i) CK(hipMalloc(&ctx->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem)));
+    for (int i = 0; i < 2; ++i) {
+        CK(hipMalloc(&ctx->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem)));
+        CK(hipMalloc(&ctx->ghb[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem)));
+    }
+    CK(hipMalloc(&ctx->snapb, nw * 8));
     CK(hipMalloc(&ctx->gpart, (size_t)kGiantSlots * std::max(ctx->coop_grid, 1) * 8));
     CK(hipMalloc(&ctx->gdbg, (size_t)2 * std::max(ctx->coop_grid, 1) * 4));
     size_t bytes = 0;
@@ -450,6 +459,13 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.q1 = ctx->q[1];
         ga.h0 = ctx->gh[0];
         ga.h1 = ctx->gh[1];
+        ga.snapb = ctx->snapb;
+        // the backward direction runs beside the forward one: its queues are
+        // the active-queue buffers, idle until the post phase
+        ga.qb0 = ctx->fq[0];
+        ga.qb1 = ctx->fq[1];
+        ga.hb0 = ctx->ghb[0];
+        ga.hb1 = ctx->ghb[1];
         ga.fq = ctx->fq[1];
         ga.gs = ctx->gstate;
         ga.part = ctx->gpart;
@@ -538,6 +554,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             local.pivot = root;
             local.fw_size = fw_count;
             local.giant_size = gs.giant;
+            local.bw_reach = gs.bw_reach;
             local.fw_levels = gs.fw_levels;
             local.bw_levels = gs.bw_levels;
             if (ctx->clock_khz > 0) {
@@ -580,7 +597,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                     fprintf(stderr, "\n");
                 }
                 for (int d = 0; d < 2; ++d) {
-                    int64_t t = d == 0 ? gs.ts[1] : gs.ts[2];
+                    int64_t t = gs.ts[1];  // both directions start after the pivot
                     const int64_t nl = std::min<int64_t>(d == 0 ? gs.fw_levels : gs.bw_levels, kLevelTrace);
                     for (int64_t l = 0; l < nl; ++l) {
                         fprintf(stderr,
@@ -645,6 +662,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         if ((rc = readback(ctx, &syncs))) return rc;
         n = ctx->h_ctl->fq_n[1];
         local.giant_size = root >= 0 ? fq0 - n : 0;  // giant = removed by post_giant
+        local.bw_reach = local.giant_size;           // engine 1: backward inside the forward set
     }
     int sel = 1;
     int64_t trim1 = 0, pat2 = 0, pat3 = 0;
@@ -1370,7 +1388,7 @@ int check_csr_device(int32_t V, int64_t E, const int32_t* fwb, const int32_t* fw
 int64_t solve_state_bytes(int64_t V, int64_t E) {
     // per-vertex solve state (labels, bitmaps, queues, WCC and mice arrays)
     // and the hub work items of both kinds of traversal
-    return 80 * (V + 1) + 2 * (giant_hub_capacity(E) + 2 * (E / kHubChunk) + 64) * (int64_t)sizeof(HubItem);
+    return 88 * (V + 1) + 2 * (2 * giant_hub_capacity(E) + 2 * (E / kHubChunk) + 64) * (int64_t)sizeof(HubItem);
 }
 
 int solve_arena(ispan_ctx* ctx, const ArenaView& av, int32_t n, const int64_t* vb, const int64_t* ve,
@@ -1725,7 +1743,11 @@ size_t carve_solve_state(ispan_ctx* c, unsigned char* base, int32_t V, int64_t E
     take(c->qbuf, nv * 4);
     take(c->part, (size_t)c->grid * sizeof(PivotPart));
     take(c->snap, nw * 8);
-    for (int i = 0; i < 2; ++i) take(c->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem));
+    for (int i = 0; i < 2; ++i) {
+        take(c->gh[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem));
+        take(c->ghb[i], (size_t)giant_hub_capacity(E) * sizeof(HubItem));
+    }
+    take(c->snapb, nw * 8);
     take(c->gpart, (size_t)kGiantSlots * std::max(c->coop_grid, 1) * 8);
     take(c->gdbg, (size_t)2 * std::max(c->coop_grid, 1) * 4);
     unsigned char* st = nullptr;
