@@ -535,7 +535,7 @@ __device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root
 // once their input gives every wave at least this many vertices.
 constexpr int kLocalPerWave = 16;
 
-// Top-down: edges per lane per step.// Top-down: edges per lane per step.  One measured fastest: two let their
+// Top-down: edges per lane per step.  One measured fastest: two let their
 // load chains overlap, but the registers they hold push the kernel into
 // spills on other paths.
 constexpr int kTK = 1;
