@@ -87,7 +87,8 @@ struct GiantState {
     int64_t tail_rounds;  // label-propagation sweeps (wcc_mode 1) or colouring rounds (2)
     int64_t tail_sccs;    // colouring: SCCs it labelled
     int64_t bw_reach;     // vertices the backward traversal reached (the giant and what reaches it)
-    int64_t pad[2];
+    int64_t unlabelled;   // with unl_checked: vertices the kernel left at -1 (its final check)
+    int64_t unl_checked;  // 1: the kernel labelled everything it was given and counted `unlabelled`
     RLine gacc[32];     // barrier reductions: per arrival group
     RLine tacc;         // barrier reductions: over groups
     RLine res[2];       // barrier reductions: result of generation g in res[g & 1]

@@ -1317,7 +1317,7 @@ __device__ __forceinline__ bool colour_scc(Ctx& c, int& cur, int64_t n, int s, i
 }
 
 // Trims and WCC after the giant.  Returns false after a failed barrier.
-__device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
+__device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64_t& n_left) {
     const GiantArgs& a = c.a;
     GiantState* gs = a.gs;
     int cur = 1, s = 0;
@@ -1436,6 +1436,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n) {
         gs->wcc_done = wcc;
         a.fq_n_ctl[cur] = (int32_t)n;
     }
+    n_left = n;
     return true;
 }
 
@@ -1650,6 +1651,9 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
             }
             sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
             if (g) st_agent(&a.act[w], a0 & ~g);
+            // the vertices left at -1 below: only they can still be unlabelled
+            // at the end (the traversal no longer reads this bitmap)
+            st_agent(&a.snap[w], a0 & ~g);
             cnt += __popcll(a0 & ~g);
             gcnt += __popcll(g);
         }
@@ -1726,7 +1730,28 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     }
     if (a.rest) {
         if (!gsync(gs, c.G, sm, a.dbg)) return;  // queue written by every block
-        if (!rest_phase(c, fq0, tot)) return;
+        int64_t n_left = 0;
+        if (!rest_phase(c, fq0, tot, n_left)) return;
+        if (n_left == 0) {
+            // Every label is final (the rest phase's last barrier acquired
+            // them).  The post phase gave every selected vertex outside the
+            // post-time active set (`snap`) a label >= 0, so only that set
+            // is checked for -1, each block over its own word range.
+            int64_t left = 0;
+            for (int64_t w = wb + c.tid; w < we; w += kGBlock) {
+                uint64_t m = a.snap[w];
+                while (m) {
+                    const int64_t v = w * 64 + __ffsll((unsigned long long)m) - 1;
+                    left += a.lab[v] < 0;
+                    m &= m - 1;
+                }
+            }
+            left = block_sum(left, sm);
+            if (c.tid == 0) {
+                if (left) atomicAdd(reinterpret_cast<unsigned long long*>(&gs->unlabelled), (unsigned long long)left);
+                if (c.b == 0) gs->unl_checked = 1;
+            }
+        }
         if (stamp) gs->ts[5] = wall_clock64();
     }
 }

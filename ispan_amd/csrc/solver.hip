@@ -509,11 +509,14 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
             if (ga1) CK(hipEventRecord(ga1, s));
             CK(hipMemcpyAsync(ctx->h_gstate, ctx->gstate, ctx->level_trace ? sizeof(GiantState) : kGiantStateHead,
                               hipMemcpyDeviceToHost, s));
-            // the kernel normally labels every vertex: count the unlabelled
-            // ones now, so that one host wait covers the solve
-            CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
+            // the kernel normally labels every vertex and then checks, itself,
+            // that none is left at -1 (unl_checked): one host wait covers the
+            // solve and its check
             if ((rc = readback(ctx, &syncs))) return rc;
+            ctx->h_ctl->unlabelled = (int32_t)ctx->h_gstate->unlabelled;
+            bool tail = false;
             if (!ctx->h_gstate->err.v && ctx->h_gstate->n_final > 0 && (P->wcc_mode == 1 || P->wcc_mode == 2)) {
+                tail = true;
                 // active vertices are left: the WCC by label propagation, or
                 // the colouring SCC method, on the device (one more launch
                 // and wait, as the mice phase would take)
@@ -612,7 +615,9 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                 }
             }
             rest_done = true;
-            counted = gs.n_final == 0;
+            // k_tail's labels are counted by the scan after it; otherwise the
+            // kernel's own check counts when nothing was left for the host
+            counted = tail ? gs.n_final == 0 : gs.unl_checked != 0;
             ctx->prof.edges[0] = gs.work[0];
             ctx->prof.edges[2] = gs.work[1];
             ctx->prof.found[0] = gs.work[2];

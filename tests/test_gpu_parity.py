@@ -234,10 +234,14 @@ def test_large_top_down_frontiers(solver, engine):
     g = uniform_digraph(1_000_000, 4_000_000, 11)
     st = check(solver, g, engine=engine, alpha=1, td_local=0)
     assert st.fw_levels >= 10 and st.giant_size > 900_000
-    # with in-place expansion the large levels are walked by the waves that
-    # found them: fewer levels, the same labels
+    # with in-place expansion (engine 0; engine 1 ignores td_local) the large
+    # levels are walked by the waves that found them: fewer levels, the same
+    # labels.  A bottom-up level's finds serve as parents within that level,
+    # so level counts of two solves can differ by one: engine 1 compares none.
     st2 = check(solver, g, engine=engine, alpha=1)
-    assert st2.fw_levels <= st.fw_levels and st2.giant_size == st.giant_size
+    assert st2.giant_size == st.giant_size
+    if engine == 0:
+        assert st2.fw_levels < st.fw_levels
 
 
 @pytest.mark.parametrize("engine", [0, 1])
