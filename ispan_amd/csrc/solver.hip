@@ -511,8 +511,12 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                               hipMemcpyDeviceToHost, s));
             // the kernel normally labels every vertex and then checks, itself,
             // that none is left at -1 (unl_checked): one host wait covers the
-            // solve and its check
-            if ((rc = readback(ctx, &syncs))) return rc;
+            // solve and its check.  Everything the host reads now is in the
+            // state; the control block (cleared by the launch) is copied only
+            // by the readbacks of later launches.
+            CK(hipStreamSynchronize(s));
+            ++syncs;
+            *ctx->h_ctl = Ctl{};
             ctx->h_ctl->unlabelled = (int32_t)ctx->h_gstate->unlabelled;
             bool tail = false;
             if (!ctx->h_gstate->err.v && ctx->h_gstate->n_final > 0 && (P->wcc_mode == 1 || P->wcc_mode == 2)) {
