@@ -625,7 +625,11 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
         int tile = kGBlock;
         while (tile > kWave && (int64_t)(tile / 2) * c.G >= n) tile /= 2;
         const int64_t ntiles = (n + tile - 1) / tile;
-        for (int64_t u = c.b; u < ntiles + nh; u += c.G) {
+        // the backward direction's tiles start half the grid away, so that
+        // when both directions' levels are small (fewer tiles than blocks)
+        // they run on different blocks at the same time
+        const int64_t b0 = BW ? ((int64_t)c.b + c.G / 2) % c.G : c.b;
+        for (int64_t u = b0; u < ntiles + nh; u += c.G) {
             if (u < ntiles) {
                 // a tile: queue entries on threads 0..tile-1
                 const int64_t i = u * tile + c.tid;
