@@ -111,6 +111,9 @@ struct GiantState {
     // with GiantArgs::trace: latest clock of any block after the post
     // phase's label pass, its barrier, and the active queue rebuild
     int64_t post_cp[3];
+    // the launch that last cleared this state (GiantArgs::epoch); never
+    // cleared itself, and 0 when the state is allocated
+    int64_t epoch;
 };
 constexpr size_t kGiantStateHead = offsetof(GiantState, lv_ts);
 
@@ -171,6 +174,12 @@ struct GiantArgs {
     int32_t td_local;         // top-down to exhaustion: steps a wave expands its own finds in place (0: off)
     uint64_t* ckey;           // colouring: (priority, colour) per vertex
     int32_t* aux;             // colouring: smallest member per SCC root
+    // set by launch_giant: block 0 clears `state_words` words of gs and
+    // `ctl_words` of ctl, then publishes `epoch` (unique per launch, never 0)
+    // in gs->epoch; the other blocks start once they read it
+    Ctl* ctl;
+    int64_t epoch;
+    int32_t state_words, ctl_words;
 };
 
 constexpr int kGiantSlots = 16;
@@ -183,8 +192,10 @@ inline int64_t giant_hub_capacity(int64_t E) { return E / kGiantChunk + E / kGia
 int giant_max_blocks_per_cu();
 // GiantArgs::bu_mult for a grid of `grid` blocks
 int64_t giant_bu_mult(int grid);
-// zeroes a.gs and ctl, records `before` (when given), then launches k_giant
-// cooperatively
+// records `before` (when given), then launches k_giant cooperatively; the
+// kernel clears a.gs and ctl itself (a kernel queued in front of a
+// cooperative launch costs a cross-queue wait).  gs->epoch must not hold an
+// epoch of this process that is still to come: 0 at allocation does.
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before);
 // after launch_giant on the same stream, for a.wcc_mode 1 or 2 when the giant
 // phase left active vertices: the WCC by label propagation, or the colouring

@@ -11,6 +11,7 @@
 // invalidates the caches, and with device-coherent loads after one that does
 // not; device-coherent loads are also kept for what other blocks change
 // during the same phase.
+#include <atomic>
 #include "common.h"
 #include "device_util.h"
 
@@ -1446,14 +1447,48 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64
 
 }  // namespace
 
+// Start of a k_giant launch: block 0 clears the state and the control block
+// (device-coherent stores, completed before the epoch is published); the
+// other blocks wait, bounded, for the launch's epoch and then acquire.
+// Returns false after a timed-out wait.
+__device__ __forceinline__ bool launch_start(const GiantArgs& a, Smem& sm) {
+    GiantState* gs = a.gs;
+    if (blockIdx.x == 0) {
+        int64_t* p = reinterpret_cast<int64_t*>(gs);
+        for (int i = threadIdx.x; i < a.state_words; i += kGBlock) st_agent(&p[i], (int64_t)0);
+        int64_t* q = reinterpret_cast<int64_t*>(a.ctl);
+        for (int i = threadIdx.x; i < a.ctl_words; i += kGBlock) st_agent(&q[i], (int64_t)0);
+        wait_vm();
+        __syncthreads();
+        if (threadIdx.x == 0) st_agent(&gs->epoch, a.epoch);
+        return true;
+    }
+    if (threadIdx.x == 0) {
+        int ok = 1, spins = 0;
+        while (load_agent(&gs->epoch) != a.epoch) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > kSpinCap) {
+                atomicOr(&gs->err.v, 1);
+                ok = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        sm.ok = ok;
+    }
+    __syncthreads();
+    return sm.ok;
+}
+
 __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     __shared__ Smem sm;
     if (threadIdx.x == 0) {
         a.dbg[blockIdx.x] = 0;
         a.dbg[gridDim.x + blockIdx.x] = 0;
         sm.nbar = 0;
-        sm.gbase = 0;  // the state is cleared before every launch
+        sm.gbase = 0;  // block 0 clears the state (gen included) before any block starts
     }
+    if (!launch_start(a, sm)) return;
     Ctx c{a, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, lane_id(), wave_id(), sm, 0, 0, 0, 0};
     GiantState* gs = a.gs;
     const int64_t total_waves = (int64_t)c.G * kW;
@@ -1815,13 +1850,6 @@ int giant_max_blocks_per_cu() {
     return nb;
 }
 
-// Clears the cooperative kernel's state and the solve's control block in one
-// launch (both are a few KB).
-__global__ void k_zero_state(int64_t* gs, int64_t ngs, int64_t* ctl, int64_t nctl) {
-    for (int64_t i = threadIdx.x; i < ngs; i += blockDim.x) gs[i] = 0;
-    for (int64_t i = threadIdx.x; i < nctl; i += blockDim.x) ctl[i] = 0;
-}
-
 int64_t giant_bu_mult(int grid) {
     const int64_t t = (int64_t)grid * kW;
     int64_t m = 2654435761ll % t;  // Knuth's multiplicative hash constant, then the next value coprime to t
@@ -1837,14 +1865,16 @@ int64_t giant_bu_mult(int grid) {
 }
 
 hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, hipEvent_t before) {
-    static_assert(kGiantStateHead % 8 == 0 && sizeof(Ctl) % 8 == 0, "state sizes");
-    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(1024), 0, s, reinterpret_cast<int64_t*>(a.gs),
-                       (int64_t)((a.trace ? sizeof(GiantState) : kGiantStateHead) / 8), reinterpret_cast<int64_t*>(ctl),
-                       (int64_t)(sizeof(Ctl) / 8));
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    static_assert(kGiantStateHead % 8 == 0 && sizeof(Ctl) % 8 == 0 && offsetof(GiantState, epoch) % 8 == 0,
+                  "state sizes");
+    static std::atomic<int64_t> epochs{0};
+    hipError_t e;
     if (before && (e = hipEventRecord(before, s)) != hipSuccess) return e;
     GiantArgs copy = a;
+    copy.ctl = ctl;
+    copy.epoch = ++epochs;
+    copy.state_words = (int32_t)((a.trace ? offsetof(GiantState, epoch) : kGiantStateHead) / 8);
+    copy.ctl_words = (int32_t)(sizeof(Ctl) / 8);
     void* args[] = {&copy};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args, 0, s);
 }

@@ -941,6 +941,8 @@ int ispan_ctx_create(ispan_ctx** out, int device) {
         if (const char* env = getenv("ISPAN_LEVEL_TRACE")) ctx->level_trace = atoi(env) != 0;
     }
     if (hipMalloc(&ctx->gstate, sizeof(GiantState)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
+    // epoch 0: no launch of this process has cleared it (the kernel clears the rest)
+    if (hipMemset(ctx->gstate, 0, sizeof(GiantState)) != hipSuccess) return fail(ISPAN_ERR_HIP);
     if (hipHostMalloc(&ctx->h_gstate, sizeof(GiantState), hipHostMallocDefault) != hipSuccess)
         return fail(ISPAN_ERR_NOMEM);
     if (hipMalloc(&ctx->ctl, sizeof(Ctl)) != hipSuccess) return fail(ISPAN_ERR_NOMEM);
@@ -1814,7 +1816,10 @@ int ispan_scc_detection_noalloc(int32_t V, int64_t E, const int32_t* d_fw_beg, c
     c.bwa = d_bw_adj;
     if ((rc = check_bound(&c, &P))) return rc;
     if (V > 0) {
-        if (launch_first(c.fwb, c.fwa, V, c.fw_first, c.grid, c.stream) != hipSuccess ||
+        // the kernel's state sits wherever this call's layout puts it: its
+        // epoch word must not hold a value a launch is still to publish
+        if (launch_fill_i32(reinterpret_cast<int32_t*>(&c.gstate->epoch), 0, 2, 1, c.stream) != hipSuccess ||
+            launch_first(c.fwb, c.fwa, V, c.fw_first, c.grid, c.stream) != hipSuccess ||
             launch_first(c.bwb, c.bwa, V, c.bw_first, c.grid, c.stream) != hipSuccess)
             return ISPAN_ERR_HIP;
     }
