@@ -1090,18 +1090,25 @@ __device__ __forceinline__ bool traverse_joint(Ctx& c, int32_t root, int64_t set
 // One trim-1 sweep (trim_1_from_fq_gfq, src/trim_1_gfq.h:365-458) over
 // fin[0, n): act is updated in place, survivors are pushed to fout.
 // Returns the number this block trimmed.
-__device__ __forceinline__ int64_t trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount) {
+// With lo >= 0 the block sweeps only fin[lo, lo + n), entries it wrote
+// itself, so no barrier has to make other blocks' entries visible first.
+__device__ __forceinline__ int64_t trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount,
+                                              int64_t lo = -1) {
     const GiantArgs& a = c.a;
     WaveQueue wq{c.sm.wbuf[c.wave], 0};
     const uint64_t* act = a.act;
     auto live = [act](int32_t w, int32_t self) { return w != self && bit_of(act, w); };
     int trimmed = 0, scanned = 0;
-    for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < n; ib += (int64_t)c.G * kGBlock) {
+    const bool local = lo >= 0;
+    const int64_t first = local ? c.wave * kWave : (int64_t)c.b * kGBlock + c.wave * kWave;
+    const int64_t step = local ? kGBlock : (int64_t)c.G * kGBlock;
+    if (local) fin += lo;
+    for (int64_t ib = first; ib < n; ib += step) {
         const int64_t i = ib + c.lane;
         int32_t v = 0;
         bool alive = false;
         if (i < n) {
-            v = fin[i];
+            v = local ? load_agent(&fin[i]) : fin[i];
             alive = bit_of(a.act, v);
         }
         const bool out = wave_any_edge(alive, alive ? a.fwb[v] : 0, alive ? a.fwb[v + 1] : 0, v, a.fwa, live, scanned);
@@ -1322,7 +1329,11 @@ __device__ __forceinline__ bool colour_scc(Ctx& c, int& cur, int64_t n, int s, i
 }
 
 // Trims and WCC after the giant.  Returns false after a failed barrier.
-__device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64_t& n_left) {
+// own_lo / own_n: the slice of the active queue this block wrote; the first
+// trim sweep takes only it, so the queue needs no barrier before it (one is
+// taken when no sweep comes first).
+__device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64_t own_lo, int64_t own_n,
+                                           int64_t& n_left) {
     const GiantArgs& a = c.a;
     GiantState* gs = a.gs;
     int cur = 1, s = 0;
@@ -1337,7 +1348,8 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64
     auto sweep = [&]() -> bool {
         if (c.b == 0 && c.tid == 0)
             __hip_atomic_store(&gs->fqn[(s + 1) % 3].v, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int64_t t = trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v);
+        const int64_t t = s == 0 ? trim_sweep(c, buf(cur), own_n, buf(cur ^ 1), &gs->fqn[s % 3].v, own_lo)
+                                 : trim_sweep(c, buf(cur), n, buf(cur ^ 1), &gs->fqn[s % 3].v);
         Red r;
         if (!gsync<true>(gs, c.G, c.sm, a.dbg, Red{t, 0, 0, 0}, &r, 1)) return false;
         trim1 += r.s0;
@@ -1355,6 +1367,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64
         ++it;
     }
     bool wcc = false;
+    if (n > 0 && s == 0 && !gsync(gs, c.G, c.sm, a.dbg)) return false;  // no sweep: the queue's barrier
     if (n > 0) {
         // P8-P10: pattern detection against the current act, then commit.
         // Over the active queue (trim_*_from_fq, src/trim_2_3.h:146-283,
@@ -1730,6 +1743,11 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         off = block_sum(x0, sm);
         tot = block_sum(xt, sm);
     }
+    // this block's slice of the queue, the active vertices of its own words.
+    // The first trim sweep takes it alone (no barrier before it): a sweep
+    // clears act bits only of the vertices it sweeps, so no block's sweep
+    // touches the words another block is still listing.
+    const int64_t own_lo = off;
     for (int64_t base = wb; base < we; base += kGBlock) {
         const int64_t w = base + c.tid;
         const uint64_t m = w < we ? a.act[w] : 0ull;
@@ -1768,9 +1786,9 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         }
     }
     if (a.rest) {
-        if (!gsync(gs, c.G, sm, a.dbg)) return;  // queue written by every block
+        __syncthreads();  // the block's queue slice is written
         int64_t n_left = 0;
-        if (!rest_phase(c, fq0, tot, n_left)) return;
+        if (!rest_phase(c, fq0, tot, own_lo, off - own_lo, n_left)) return;
         if (n_left == 0) {
             // Every label is final (the rest phase's last barrier acquired
             // them).  The post phase gave every selected vertex outside the
