@@ -109,7 +109,7 @@ struct GiantState {
     // level of the forward traversal, and the adjacency entries it scanned
     int64_t bu1_work[1024], bu1_scan[1024];
     // with GiantArgs::trace: latest clock of any block after the post
-    // phase's label pass, its barrier, and the active queue rebuild
+    // phase's count pass, its barrier, and the labels and active queue
     int64_t post_cp[3];
     // the launch that last cleared this state (GiantArgs::epoch); never
     // cleared itself, and 0 when the state is allocated

@@ -1660,23 +1660,6 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         bw_reach = b.visited + seeded;
         fw_levels = f.L;
         bw_levels = b.L;
-        if (!multi) {
-            // the giant's smallest id (its label): no larger than the pivot,
-            // which the giant contains, so only words up to the pivot's count
-            int32_t mn = INT32_MAX;
-            for (int64_t w = (int64_t)c.b * kGBlock + c.tid; w <= (root >> 6); w += (int64_t)c.G * kGBlock) {
-                const uint64_t gw = a.fvis[w] & a.bvis[w];
-                if (gw) mn = min(mn, (int32_t)(w * 64 + __ffsll((unsigned long long)gw) - 1));
-            }
-            mn = wave_min(mn);
-            if (c.lane == 0) sm.tmp[c.wave] = mn;
-            __syncthreads();
-            mn = lane_get(row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); }), 15);
-            __syncthreads();
-            Red r;
-            if (!gsync<true>(gs, c.G, sm, a.dbg, Red{0, 0, 0, (int64_t)INT32_MAX - mn}, &r, 8)) return;
-            gmin = (int32_t)((int64_t)INT32_MAX - r.key);
-        }
     }
     if (stamp) {
         if (!traversed) gs->ts[2] = wall_clock64();
@@ -1685,41 +1668,39 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
 
     // ---- post: giant labels, act minus giant, every label written once;
     // each block owns a contiguous word range so the rebuilt active queue
-    // comes out ascending without atomics.
+    // comes out ascending without atomics.  First the block's words are
+    // narrowed and counted, and the giant's smallest id (its label: no larger
+    // than the pivot, which the giant contains) is found among words up to
+    // the pivot's; the barrier after the counts reduces it as its key.
     const int64_t per = (a.nwords + c.G - 1) / c.G;
     const int64_t wb = min((int64_t)c.b * per, a.nwords), we = min(wb + per, a.nwords);
+    const bool want_min = traversed && !multi;
     int64_t cnt = 0, gcnt = 0;
-    // one word per lane: the wave loads 64 words at once, then writes their
-    // labels word by word (coalesced 256 B stores, no loads in the loop)
-    for (int64_t base = wb + (int64_t)c.wave * kWave; base < we; base += (int64_t)kW * kWave) {
-        const int64_t w = base + c.lane;
-        uint64_t a0 = 0, g = 0, sw = 0;
-        if (w < we) {
-            a0 = a.act[w];
-            if (traversed) {  // the giant: reached both ways; bvis keeps only it
-                const uint64_t bw = a.bvis[w];
-                g = a.fvis[w] & bw;
-                if (g != bw) st_agent(&a.bvis[w], g);
-            }
-            sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
-            if (g) st_agent(&a.act[w], a0 & ~g);
-            // the vertices left at -1 below: only they can still be unlabelled
-            // at the end (the traversal no longer reads this bitmap)
-            st_agent(&a.snap[w], a0 & ~g);
-            cnt += __popcll(a0 & ~g);
-            gcnt += __popcll(g);
+    int32_t mn = INT32_MAX;
+    for (int64_t w = wb + c.tid; w < we; w += kGBlock) {
+        const uint64_t a0 = a.act[w];
+        uint64_t g = 0;
+        if (traversed) {  // the giant: reached both ways; bvis keeps only it
+            const uint64_t bw = a.bvis[w];
+            g = a.fvis[w] & bw;
+            if (g != bw) st_agent(&a.bvis[w], g);
         }
-        const int nw = (int)min<int64_t>(kWave, we - base);
-        for (int j = 0; j < nw; ++j) {
-            const uint64_t aj = lane_get(a0, j), gj = lane_get(g, j), sj = lane_get(sw, j);
-            const int64_t v = (base + j) * 64 + c.lane;
-            if (v < a.V && ((sj >> c.lane) & 1ull)) {
-                const bool gi = (gj >> c.lane) & 1ull, ac = (aj >> c.lane) & 1ull;
-                st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (ac ? -1 : (int32_t)v));
-            }
-        }
+        if (g) st_agent(&a.act[w], a0 & ~g);
+        // the vertices left at -1 below: only they can still be unlabelled at
+        // the end (the traversal no longer reads this bitmap)
+        st_agent(&a.snap[w], a0 & ~g);
+        cnt += __popcll(a0 & ~g);
+        gcnt += __popcll(g);
+        if (want_min && g && w <= (root >> 6)) mn = min(mn, (int32_t)(w * 64 + __ffsll((unsigned long long)g) - 1));
     }
     const int64_t cnt_blk = block_sum(cnt, sm), giant_blk = block_sum(gcnt, sm);
+    if (want_min) {
+        mn = wave_min(mn);
+        if (c.lane == 0) sm.tmp[c.wave] = mn;
+        __syncthreads();
+        mn = lane_get(row_incl(sm.tmp[c.lane & 15], INT32_MAX, [](int32_t x, int32_t y) { return min(x, y); }), 15);
+        __syncthreads();
+    }
     if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[0], (unsigned long long)wall_clock64());
     if (c.tid == 0) {
         st_agent(&a.part[(int64_t)kPost * c.G + c.b], (int64_t)(cnt_blk));
@@ -1729,8 +1710,34 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
         st_agent(&a.part[(int64_t)kFoundTD * c.G + c.b], (int64_t)(c.found_td));
         st_agent(&a.part[(int64_t)kFoundBU * c.G + c.b], (int64_t)(c.found_bu));
     }
-    if (!gsync(gs, c.G, sm, a.dbg)) return;
+    {
+        Red r;
+        if (!gsync<true>(gs, c.G, sm, a.dbg, Red{0, 0, 0, mn < INT32_MAX ? (int64_t)INT32_MAX - mn : 0}, &r, 8))
+            return;
+        if (want_min) gmin = (int32_t)((int64_t)INT32_MAX - r.key);
+    }
     if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[1], (unsigned long long)wall_clock64());
+    // the labels: one word per lane, the wave loads 64 words at once (its own
+    // block's, narrowed above), then writes their labels word by word
+    // (coalesced 256 B stores, no loads in the loop)
+    for (int64_t base = wb + (int64_t)c.wave * kWave; base < we; base += (int64_t)kW * kWave) {
+        const int64_t w = base + c.lane;
+        uint64_t ac = 0, g = 0, sw = 0;
+        if (w < we) {
+            ac = a.act[w];
+            g = traversed ? a.bvis[w] : 0ull;
+            sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
+        }
+        const int nw = (int)min<int64_t>(kWave, we - base);
+        for (int j = 0; j < nw; ++j) {
+            const uint64_t aj = lane_get(ac, j), gj = lane_get(g, j), sj = lane_get(sw, j);
+            const int64_t v = (base + j) * 64 + c.lane;
+            if (v < a.V && ((sj >> c.lane) & 1ull)) {
+                const bool gi = (gj >> c.lane) & 1ull, act_v = (aj >> c.lane) & 1ull;
+                st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (act_v ? -1 : (int32_t)v));
+            }
+        }
+    }
     // exclusive prefix of the block counts, then an ordered block-local scan
     int64_t off, tot;
     {

@@ -581,7 +581,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
                         (gs.ts[1] - gs.ts[0]) * k0);
                 const double k = 1.0 / ctx->clock_khz;
                 if (gs.post_cp[0])
-                    fprintf(stderr, "[ispan] post after BW: labels %.3f ms, barrier %.3f ms, queue %.3f ms\n",
+                    fprintf(stderr, "[ispan] post after BW: counts %.3f ms, barrier %.3f ms, labels and queue %.3f ms\n",
                             (gs.post_cp[0] - gs.ts[3]) * k, (gs.post_cp[1] - gs.ts[3]) * k,
                             (gs.post_cp[2] - gs.ts[3]) * k);
                 if (gs.bu1_scan[0]) {  // first forward bottom-up level, per block and by XCC (block % 8)
