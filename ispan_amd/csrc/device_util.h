@@ -287,6 +287,35 @@ __device__ __forceinline__ Nb nb_scan(const int32_t* __restrict__ beg, const int
     return r;
 }
 
+// nb_scan by a whole wave (every lane calls it with the same x, e1, e2,
+// cap): 64 list entries per step, and within a step the lanes in list order,
+// so the count and the first two neighbours are nb_scan's.
+__device__ __forceinline__ Nb nb_scan_wave(const int32_t* __restrict__ beg, const int32_t* __restrict__ adj,
+                                           const uint64_t* __restrict__ act, int32_t x, int32_t e1, int32_t e2,
+                                           int cap) {
+    Nb r{0, -1, -1};
+    const int32_t end = beg[x + 1];
+    for (int32_t k0 = beg[x]; k0 < end; k0 += kWave) {
+        const int32_t k = k0 + (int32_t)lane_id();
+        int32_t w = -1;
+        bool ok = false;
+        if (k < end) {
+            w = adj[k];
+            ok = w != x && w != e1 && w != e2 && bit_of(act, w);
+        }
+        while (true) {
+            // the first entry (in list order) that is not one counted already
+            const uint64_t m = __ballot(ok && !(r.n >= 1 && w == r.g0) && !(r.n >= 2 && w == r.g1));
+            if (!m) break;
+            const int32_t wl = lane_get(w, __ffsll((unsigned long long)m) - 1);
+            if (r.n == 0) r.g0 = wl;
+            else if (r.n == 1) r.g1 = wl;
+            if (++r.n > cap) return r;
+        }
+    }
+    return r;
+}
+
 __device__ __forceinline__ void pattern_mark(int32_t* lab, int32_t* stamp, int32_t a, int32_t b, int32_t c, int size) {
     int32_t m = min(a, b);
     if (size == 3) m = min(m, c);
@@ -302,10 +331,21 @@ __device__ __forceinline__ void pattern_mark(int32_t* lab, int32_t* stamp, int32
 
 // Checks vertex a against the size-2 pattern, then the size-3 cycle, then
 // the size-3 hub, on out-edges and then in-edges; marks the first match.
+// WAVE: the whole wave checks one vertex (a uniform, every lane calls), its
+// lists scanned 64 entries at a time; otherwise each lane checks its own.
+template <bool WAVE = false>
 __device__ __forceinline__ void pattern_check(const int32_t* __restrict__ fwb, const int32_t* __restrict__ fwa,
                                               const int32_t* __restrict__ bwb, const int32_t* __restrict__ bwa,
                                               int32_t* lab, int32_t* stamp, const uint64_t* __restrict__ act,
                                               int32_t a) {
+    auto nb_scan = [](const int32_t* beg, const int32_t* adj, const uint64_t* act, int32_t x, int32_t e1, int32_t e2,
+                      int cap) {
+        if constexpr (WAVE) return nb_scan_wave(beg, adj, act, x, e1, e2, cap);
+        else return dev::nb_scan(beg, adj, act, x, e1, e2, cap);
+    };
+    auto pattern_mark = [](int32_t* lab, int32_t* stamp, int32_t a, int32_t b, int32_t c, int size) {
+        if (!WAVE || lane_id() == 0) dev::pattern_mark(lab, stamp, a, b, c, size);
+    };
     for (int dir = 0; dir < 2; ++dir) {
         const int32_t* beg = dir == 0 ? fwb : bwb;
         const int32_t* adj = dir == 0 ? fwa : bwa;

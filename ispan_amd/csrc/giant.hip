@@ -1329,6 +1329,10 @@ __device__ __forceinline__ bool colour_scc(Ctx& c, int& cur, int64_t n, int s, i
 }
 
 // Trims and WCC after the giant.  Returns false after a failed barrier.
+// Pattern checks: a vertex with more entries than this in either list is
+// checked by a whole wave instead of one lane.
+constexpr int kPatternLong = 32;
+
 // own_lo / own_n: the slice of the active queue this block wrote; the first
 // trim sweep takes only it, so the queue needs no barrier before it (one is
 // taken when no sweep comes first).
@@ -1391,7 +1395,30 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64
                 }
             }
         };
-        each_active([&](int32_t v) { pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v); });
+        // A lane checks its own vertex, except one with a long list (a lane
+        // would walk it entry by entry): those are queued in the wave's LDS
+        // buffer and checked afterwards by the whole wave, 64 entries a step.
+        {
+            int32_t* lq = c.sm.wbuf[c.wave];
+            int32_t& nlq = c.sm.wcnt[0][c.wave];
+            if (c.lane == 0) nlq = 0;
+            wave_sync();
+            each_active([&](int32_t v) {
+                const int32_t dv = max(a.fwb[v + 1] - a.fwb[v], a.bwb[v + 1] - a.bwb[v]);
+                if (dv > kPatternLong) {
+                    const int slot = atomicAdd(&nlq, 1);
+                    if (slot < kWaveBuf) {
+                        lq[slot] = v;
+                        return;
+                    }
+                }
+                pattern_check(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, v);
+            });
+            wave_sync();
+            const int nl = min(nlq, kWaveBuf);
+            for (int j = 0; j < nl; ++j)
+                pattern_check<true>(a.fwb, a.fwa, a.bwb, a.bwa, a.lab, a.stamp, a.act, lq[j]);
+        }
         if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         int c2 = 0, c3 = 0;
         each_active([&](int32_t v) {

@@ -53,6 +53,71 @@ def test_pattern_graphs(solver, name, d, wcc_mode, trim_scope):
         assert st.trim3 == 3
 
 
+def long_list_patterns(seed: int):
+    """A dense strongly connected core (the giant) and size-2/3 pattern groups
+    whose vertices have long lists once the core is removed: 40 out-edges into
+    the core, or 40 in-edges from sources that trim-1-first removes, plus
+    duplicate edges and self-loops.  Every pattern vertex keeps out*in below
+    the core's, so the pivot is in the core.  Returns the graph and the
+    numbers of pair and triple vertices."""
+    rng = np.random.default_rng(seed)
+    n_core = 2000
+    edges = [(i, (i + 1) % n_core) for i in range(n_core)]
+    edges += [(int(u), int(v)) for u, v in rng.integers(0, n_core, size=(n_core * 20, 2)) if u != v]
+    nxt = n_core
+    pairs = triples = 0
+
+    def tails(x, long_in):
+        nonlocal nxt
+        if long_in:  # 40 sources into x, one edge from x into the core
+            for _ in range(40):
+                edges.append((nxt, x))
+                nxt += 1
+            edges.append((x, int(rng.integers(0, n_core))))
+        else:  # 40 edges from x into the core, some repeated
+            for w in rng.integers(0, n_core, size=40):
+                edges.append((x, int(w)))
+            edges.append((x, int(rng.integers(0, n_core))))
+        edges.append((x, x))
+
+    for k in range(30):
+        long_in = k % 2 == 1
+        kind = k % 3
+        if kind == 0:  # pair a <-> b
+            a_, b_ = nxt, nxt + 1
+            nxt += 2
+            edges += [(a_, b_), (b_, a_), (a_, b_)]
+            for x in (a_, b_):
+                tails(x, long_in)
+            pairs += 2
+        elif kind == 1:  # cycle a -> b -> c -> a
+            a_, b_, c_ = nxt, nxt + 1, nxt + 2
+            nxt += 3
+            edges += [(a_, b_), (b_, c_), (c_, a_)]
+            for x in (a_, b_, c_):
+                tails(x, long_in)
+            triples += 3
+        else:  # hub a <-> b, a <-> c
+            a_, b_, c_ = nxt, nxt + 1, nxt + 2
+            nxt += 3
+            edges += [(a_, b_), (b_, a_), (a_, c_), (c_, a_)]
+            for x in (a_, b_, c_):
+                tails(x, long_in)
+            triples += 3
+    return O.graph_from_edges(nxt, edges), pairs, triples
+
+
+@pytest.mark.parametrize("wcc_mode,trim_scope", [(0, 0), (2, 0), (0, 1)])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_patterns_with_long_lists(solver, seed, wcc_mode, trim_scope):
+    """Pattern vertices with more than 32 entries in a list are checked by a
+    whole wave; the detections and labels are those of the per-lane check."""
+    g, pairs, triples = long_list_patterns(seed)
+    st = check(solver, g, wcc_mode=wcc_mode, trim_scope=trim_scope)
+    assert st.giant_size == 2000
+    assert st.trim2 == pairs and st.trim3 == triples
+
+
 # --------------------------------------------------------- random graphs
 
 PARAMS = [dict(), dict(wcc_mode=1), dict(pivot_wrap32=1), dict(alpha=1, beta=1, gamma=0),
