@@ -827,11 +827,19 @@ int orc_ispan(int32_t V, const int32_t* fwb, const int32_t* fwa, const int32_t* 
                                 if (nr == r) break;
                                 r = nr;
                             }
+                            // compress v's chain towards r, never lowering a
+                            // colour.  The walk stops at a root (colour ==
+                            // id) and where another thread has already raised
+                            // the chain past r: colours only grow, so climbing
+                            // on from there could end at a root above r and
+                            // spin on it (the reference writes root
+                            // unconditionally instead, :560-565).
                             int32_t x = v;
                             while (x != r) {
                                 int32_t nx = ld(&color[x]);
                                 if (nx == r) break;
                                 if (nx < r) st(&color[x], r);
+                                if (nx == x || nx > r) break;
                                 x = nx;
                             }
                         }
