@@ -127,8 +127,19 @@ typedef struct ispan_stats {
                              0 runs it beside the forward one over every
                              active vertex (the giant is the intersection);
                              engine 1 inside the forward set (= giant_size) */
-    int64_t reserved[7];
+    int64_t max_wcc;      /* largest WCC left for the mice phase             */
+    int64_t coloured;     /* engine 0: vertices the colouring method took
+                             instead of the mice phase, because a WCC had
+                             more than ISPAN_MICE_MAX_WCC vertices           */
+    int64_t reserved[5];
 } ispan_stats;
+
+/* Engine 0 hands the vertices left after the trims to the colouring method
+ * (as wcc_mode 2) instead of the mice phase when one of their WCCs has more
+ * vertices than this: FW-BW per WCC searches forward from each smallest
+ * unlabelled vertex in turn, which on a long WCC of small SCCs costs its size
+ * per SCC.  Labels do not change. */
+#define ISPAN_MICE_MAX_WCC 1024
 
 typedef struct ispan_ctx ispan_ctx;
 

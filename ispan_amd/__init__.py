@@ -99,7 +99,7 @@ class Stats(C.Structure):
                 ("trim1", C.c_int64), ("trim2", C.c_int64), ("trim3", C.c_int64), ("n_wcc", C.c_int64),
                 ("mice", C.c_int64), ("fw_levels", C.c_int64), ("bw_levels", C.c_int64), ("n_scc", C.c_int64),
                 ("host_syncs", C.c_int64), ("engine_used", C.c_int64), ("bw_reach", C.c_int64),
-                ("reserved", C.c_int64 * 7)]
+                ("max_wcc", C.c_int64), ("coloured", C.c_int64), ("reserved", C.c_int64 * 5)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("ms", "reserved")}

@@ -34,7 +34,7 @@ struct alignas(128) Ctl {
     int32_t unlabelled;     // final check
     uint32_t done_blocks;   // last-block-done counter (pivot reduction)
     int32_t bad_csr;        // CSR validation failures
-    int32_t pad0;
+    int32_t max_wcc;        // largest WCC entering the mice phase
     int64_t fw_count;       // |FW set| (popcount of fvis)
     int64_t fw_edges;       // statistics pass
     int64_t bw_edges;
@@ -286,9 +286,11 @@ hipError_t launch_lp_round(const int32_t* fwb, const int32_t* fwa, const int32_t
 hipError_t launch_mice_keys(const int32_t* fq, const int32_t* wcc, int32_t* stamp, uint64_t* keys, Ctl* ctl, int sel,
                             int grid, hipStream_t s);
 hipError_t launch_mice_starts(const uint64_t* keys, int32_t n, int32_t* starts, Ctl* ctl, int grid, hipStream_t s);
+// max_wcc > 0: does nothing when a WCC has more vertices than that
+// (ctl->max_wcc, from launch_mice_starts); the caller labels them otherwise
 hipError_t launch_mice(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa, int32_t* lab,
                        int32_t* stamp, const uint64_t* keys, int32_t n, const int32_t* starts, Ctl* ctl,
-                       int32_t* qbuf, int grid, hipStream_t s);
+                       int32_t* qbuf, int32_t max_wcc, int grid, hipStream_t s);
 hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, const uint64_t* sel, Ctl* ctl, int grid,
                                    hipStream_t s);
 hipError_t launch_stats(const int32_t* fwb, const int32_t* bwb, const int32_t* lab, const uint64_t* fvis,

@@ -610,10 +610,23 @@ __global__ __launch_bounds__(kBlock) void k_mice_keys(const int32_t* __restrict_
     }
 }
 
+// The start of every WCC in the sorted (WCC, vertex) keys, and the largest
+// WCC's size: a start finds its WCC's end by binary search (keys ascend by
+// WCC in their high half).
 __global__ __launch_bounds__(kBlock) void k_mice_starts(const uint64_t* __restrict__ keys, int32_t n, int32_t* starts,
                                                         Ctl* ctl) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) starts[atomicAdd(&ctl->n_wcc, 1)] = (int32_t)i;
+        const uint64_t w = keys[i] >> 32;
+        if (i == 0 || w != (keys[i - 1] >> 32)) {
+            starts[atomicAdd(&ctl->n_wcc, 1)] = (int32_t)i;
+            int64_t lo = i + 1, hi = n;  // first index past the WCC
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if ((keys[mid] >> 32) == w) lo = mid + 1;
+                else hi = mid;
+            }
+            atomicMax(&ctl->max_wcc, (int32_t)(lo - i));
+        }
     }
 }
 
@@ -621,8 +634,9 @@ __global__ __launch_bounds__(kBlock) void k_mice(const int32_t* __restrict__ fwb
                                                  const int32_t* __restrict__ bwb, const int32_t* __restrict__ bwa,
                                                  int32_t* lab, int32_t* stamp, const uint64_t* __restrict__ keys,
                                                  int32_t n, const int32_t* __restrict__ starts, const Ctl* ctl,
-                                                 int32_t* qbuf) {
+                                                 int32_t* qbuf, int32_t max_wcc) {
     __shared__ int s_tail[kBlock / kWave];
+    if (max_wcc > 0 && ctl->max_wcc > max_wcc) return;  // left to the caller's other method
     const int lane = lane_id(), wave = wave_id();
     const int nw = ctl->n_wcc;
     const int waves = gridDim.x * (kBlock / kWave);
@@ -927,8 +941,8 @@ hipError_t launch_mice_starts(const uint64_t* keys, int32_t n, int32_t* starts, 
 
 hipError_t launch_mice(const int32_t* fwb, const int32_t* fwa, const int32_t* bwb, const int32_t* bwa, int32_t* lab,
                        int32_t* stamp, const uint64_t* keys, int32_t n, const int32_t* starts, Ctl* ctl,
-                       int32_t* qbuf, int grid, hipStream_t s) {
-    LAUNCH(k_mice, grid, fwb, fwa, bwb, bwa, lab, stamp, keys, n, starts, ctl, qbuf);
+                       int32_t* qbuf, int32_t max_wcc, int grid, hipStream_t s) {
+    LAUNCH(k_mice, grid, fwb, fwa, bwb, bwa, lab, stamp, keys, n, starts, ctl, qbuf, max_wcc);
 }
 
 hipError_t launch_count_unlabelled(const int32_t* lab, int32_t V, const uint64_t* sel, Ctl* ctl, int grid,

@@ -439,6 +439,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
     bool rest_done = false;
     double giant_phase_ms[4] = {0, 0, 0, 0};  // engine 0: trim-1-first+pivot, FW, BW+relabel, trims+WCC
     bool counted = false;  // ctl->unlabelled holds the count of the finished labelling
+    GiantArgs ga_tail{};   // engine 0: the giant kernel's arguments, for a colouring tail
     if (coop) {
         // P1-P6 + relabel in one cooperative kernel
         GiantArgs ga{};
@@ -495,6 +496,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.td_local = std::max(P->td_local, 0);
         ga.ckey = ctx->keys;
         ga.aux = ctx->qbuf;
+        ga_tail = ga;
         hipEvent_t ga0 = nullptr, ga1 = nullptr;
         if (P->timing >= 2 && (ga0 = pool_event(ctx)) && (ga1 = pool_event(ctx))) ctx->recs.push_back({3, ga0, ga1});
         const hipError_t le = launch_giant(ga, ctx->ctl, ctx->coop_grid, s, ga0);
@@ -756,17 +758,33 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         CK(rocprim::radix_sort_keys(ctx->sort_tmp, bytes, ctx->keys, ctx->keys_alt, (size_t)n, 0, (unsigned)end_bit,
                                     s));
         CK(launch_mice_starts(ctx->keys_alt, (int32_t)n, ctx->starts, ctx->ctl, G, s));
+        // FW-BW per WCC costs a long WCC of small SCCs its size per SCC (each
+        // search runs down the rest of it): on engine 0, past
+        // ISPAN_MICE_MAX_WCC vertices k_mice leaves everything to the
+        // colouring method, launched after the readback below shows it
         CK(launch_mice(ctx->fwb, ctx->fwa, ctx->bwb, ctx->bwa, ctx->lab, ctx->stamp, ctx->keys_alt, (int32_t)n,
-                       ctx->starts, ctx->ctl, ctx->qbuf, G, s));
+                       ctx->starts, ctx->ctl, ctx->qbuf, coop ? ISPAN_MICE_MAX_WCC : 0, G, s));
         local.mice = n;
     } else if (!coop) {
         rec(ctx, P, 8);
     }
     if (!coop || host_tail) rec(ctx, P, 9);
     if (!counted) {
-        CK(hipMemsetAsync(&ctx->ctl->unlabelled, 0, sizeof(ctx->ctl->unlabelled), s));
+        CK(launch_fill_i32(&ctx->ctl->unlabelled, 0, 1, 1, s));
         CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
         if ((rc = readback(ctx, &syncs))) return rc;
+    }
+    if (host_tail) local.max_wcc = ctx->h_ctl->max_wcc;
+    if (coop && host_tail && ctx->h_ctl->max_wcc > ISPAN_MICE_MAX_WCC) {
+        // k_mice left this rest alone: the colouring method labels it
+        GiantArgs gc = ga_tail;
+        gc.wcc_mode = 2;
+        CK(launch_tail(gc, ctx->coop_grid, s));
+        CK(launch_fill_i32(&ctx->ctl->unlabelled, 0, 1, 1, s));
+        CK(launch_count_unlabelled(ctx->lab, V, ctx->batch_sel, ctx->ctl, G, s));
+        if ((rc = readback(ctx, &syncs))) return rc;
+        local.coloured = local.mice;
+        local.mice = 0;
     }
     if (ctx->h_ctl->unlabelled) {
         ctx->err = std::to_string(ctx->h_ctl->unlabelled) + " vertices left unlabelled";

@@ -94,3 +94,20 @@ def brute_force_labels(g) -> np.ndarray:
         R |= np.outer(R[:, k], R[k, :])
     mutual = R & R.T
     return np.argmax(mutual, axis=1).astype(np.int32)  # first (smallest) mutually reachable id
+
+
+def cycle_chain(groups: int, seed: int):
+    """Groups of 1-6 vertices on a cycle each, group g pointing forward to
+    groups g+1..g+3: one weakly connected piece of small SCCs, so the WCC's
+    label propagation runs many rounds with path compression in every one."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, 7, size=groups)
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    edges = []
+    for i, k in enumerate(sizes):
+        b = int(first[i])
+        edges += [(b + j, b + (j + 1) % int(k)) for j in range(int(k))]
+        for t in (1, 2, 3):
+            if i + t < groups:
+                edges.append((b + int(rng.integers(0, k)), int(first[i + t])))
+    return O.graph_from_edges(int(first[-1]), edges)

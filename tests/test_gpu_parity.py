@@ -118,6 +118,40 @@ def test_patterns_with_long_lists(solver, seed, wcc_mode, trim_scope):
     assert st.trim2 == pairs and st.trim3 == triples
 
 
+def core_and_chain(groups: int, seed: int):
+    """A dense strongly connected core (the giant) and, apart from it, a
+    chain of small cycles (tests.graphs.cycle_chain) whose last vertex points
+    into the core: after the giant the rest is one WCC of small SCCs."""
+    rng = np.random.default_rng(seed)
+    n_core = 3000
+    core = np.concatenate([np.stack([np.arange(n_core), (np.arange(n_core) + 1) % n_core], 1),
+                           rng.integers(0, n_core, size=(n_core * 20, 2))])
+    ch = G.cycle_chain(groups, seed)
+    src = np.repeat(np.arange(ch.V, dtype=np.int64), np.diff(ch.fw_beg)) + n_core
+    dst = ch.fw_adj.astype(np.int64) + n_core
+    s_all = np.concatenate([core[:, 0], src, [n_core + ch.V - 1]]).astype(np.int32)
+    d_all = np.concatenate([core[:, 1], dst, [0]]).astype(np.int32)
+    return O.build_csr(n_core + ch.V, s_all, d_all)
+
+
+@pytest.mark.parametrize("wcc_mode", [0, 1])
+def test_long_wcc_goes_to_colouring(solver, wcc_mode):
+    """A WCC of more than ISPAN_MICE_MAX_WCC vertices left after the trims is
+    labelled by the colouring method instead of FW-BW per WCC (whose searches
+    would each run down the rest of the chain); labels are Tarjan's."""
+    g = core_and_chain(2000, 3)
+    st = check(solver, g, wcc_mode=wcc_mode)
+    assert st.giant_size == 3000
+    assert st.max_wcc > 1024 and st.coloured > 0 and st.mice == 0
+
+
+def test_small_wccs_stay_in_the_mice_phase(solver):
+    g = G.cycles_graph(5, n_cycles=400)
+    st = check(solver, g)
+    if st.n_wcc > 0:
+        assert st.max_wcc <= 1024 and st.coloured == 0 and st.mice > 0
+
+
 # --------------------------------------------------------- random graphs
 
 PARAMS = [dict(), dict(wcc_mode=1), dict(pivot_wrap32=1), dict(alpha=1, beta=1, gamma=0),

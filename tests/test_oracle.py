@@ -225,23 +225,6 @@ def test_compiled_reference_c1():
     assert set(avg) >= {"Trim", "Elephant SCC", "Mice SCC", "Total time", "FW BFS", "BW BFS"}
 
 
-def cycle_chain(groups: int, seed: int):
-    """Groups of 1-6 vertices on a cycle each, group g pointing forward to
-    groups g+1..g+3: one weakly connected piece of small SCCs, so the WCC's
-    label propagation runs many rounds with path compression in every one."""
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(1, 7, size=groups)
-    first = np.concatenate([[0], np.cumsum(sizes)])
-    edges = []
-    for i, k in enumerate(sizes):
-        b = int(first[i])
-        edges += [(b + j, b + (j + 1) % int(k)) for j in range(int(k))]
-        for t in (1, 2, 3):
-            if i + t < groups:
-                edges.append((b + int(rng.integers(0, k)), int(first[i + t])))
-    return O.graph_from_edges(int(first[-1]), edges)
-
-
 @pytest.mark.timeout(300, method="thread")  # a spin inside the C call ends the run instead of hanging it
 def test_wcc_path_compression_terminates():
     """Path compression in the restatement's WCC stops at a root and where
@@ -249,7 +232,7 @@ def test_wcc_path_compression_terminates():
     to climb from there, long chains of small SCCs could spin forever at a
     root above the target under contention."""
     for groups in (500, 600, 800, 3000):
-        g = cycle_chain(groups, groups)
+        g = G.cycle_chain(groups, groups)
         tj = O.tarjan(g)
         for threads in (2, 4, 8):
             for _ in range(3):
