@@ -211,9 +211,9 @@ int ispan_scc_detection(int32_t V, int64_t E, const int32_t* fw_beg, const int32
  * allocation).  It calls no hipMalloc/hipFree, creates no stream or event and
  * allocates no host memory on success: every buffer of the solve comes from
  * one fixed device array of 64 MiB that the library holds on each GPU it runs
- * on until the process ends.  The solve state takes about 69 bytes per vertex
- * plus 0.05 bytes per edge plus ~0.4 MB, so graphs of up to ~900,000 vertices
- * fit (R-MAT scale 19, edge factor 16 does; scale 20 does not); a larger one
+ * on until the process ends.  The solve state takes about 85 bytes per vertex
+ * plus 0.075 bytes per edge plus ~0.4 MB, so graphs of up to ~750,000 vertices
+ * fit at edge factor 16 (R-MAT scale 19 does; scale 20 does not); a larger one
  * returns ISPAN_ERR_SCRATCH before any device work.  Inputs: the CSR pair in
  * device memory (as ispan_graph_bind_device); output: d_labels (int32[V],
  * device).  Runs on the current HIP device, on hip_stream (NULL: the null

@@ -42,7 +42,6 @@ struct Smem {
     int ok;
     int32_t nbar;   // barriers this block has arrived at in this launch
     int32_t gbase;  // barrier generation when this launch started
-    int32_t wbuf[kW][kWaveBuf];
     int32_t wcnt[2][kW];    // block flushes: per-wave counts and bases of up to two counters
     int32_t wbase[2][kW];
     int32_t hv[kW][kWave];  // per-wave staged hub vertices: id, first edge, degree
@@ -50,11 +49,21 @@ struct Smem {
     int32_t hd[kW][kWave];
 };
 
+// Per-wave staging in dynamic LDS, GiantArgs::pair_cap 8-byte entries a wave:
+// top-down queue entries, or kWaveBuf vertex ids in the phases that queue
+// ids.  The capacity is chosen per launch (giant_pair_cap): only low-degree
+// graphs gain from the larger one.
+static_assert(kWaveBuf * 4 <= kPairCapMin * 8, "the id view fits the smallest staging");
+extern __shared__ uint64_t stage_lds[];
+
 struct Ctx {
     const GiantArgs& a;
     int G, b, tid, lane, wave;
     Smem& sm;
     int64_t edges_td, edges_bu, found_td, found_bu;  // work counters (thread 0's are the block's)
+    // this wave's staging (a.pair_cap entries), as 8-byte entries or as ids
+    __device__ uint64_t* stage() const { return stage_lds + wave * a.pair_cap; }
+    __device__ int32_t* stage32() const { return reinterpret_cast<int32_t*>(stage()); }
 };
 
 // Values a block contributes to the reduction a barrier carries: two sums
@@ -330,11 +339,70 @@ __device__ __forceinline__ void block_flush_both(Ctx& c, WaveQueue& wq, int32_t*
     wave_sync();
 }
 
+// A top-down queue entry: a vertex's first edge (high half) and degree (low
+// half), which is all a top-down level needs to walk the vertex's list, so
+// it loads no offsets; the claim that queues a vertex has just loaded them.
+__device__ __forceinline__ uint64_t qentry(int32_t s0, int32_t d) {
+    return ((uint64_t)(uint32_t)s0 << 32) | (uint32_t)d;
+}
+__device__ __forceinline__ int32_t qentry_start(uint64_t e) { return (int32_t)(uint32_t)(e >> 32); }
+__device__ __forceinline__ int32_t qentry_degree(uint64_t e) { return (int32_t)(uint32_t)e; }
+
+// A wave's staged top-down queue entries (in LDS) and its fill.
+struct PairQueue {
+    uint64_t* buf;
+    int n;    // wave-uniform
+    int cap;  // a power of two
+};
+
+__device__ __forceinline__ void pq_flush(PairQueue& pq, uint64_t* gq, int32_t* gcount) {
+    if (pq.n == 0) return;
+    wave_sync();
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(gcount, pq.n);
+    base = lane_get(base, 0);
+    for (int i = lane_id(); i < pq.n; i += kWave) st_agent(&gq[base + i], pq.buf[i]);
+    pq.n = 0;
+    wave_sync();
+}
+
+__device__ __forceinline__ void pq_push(PairQueue& pq, bool pred, uint64_t e, uint64_t* gq, int32_t* gcount) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int c = __popcll(m);
+    if (pq.n + c > pq.cap) pq_flush(pq, gq, gcount);
+    if (pred) pq.buf[pq.n + __popcll(m & lanemask_lt())] = e;
+    pq.n += c;
+}
+
+// block_flush_both for top-down queue entries.
+__device__ __forceinline__ void block_flush_both(Ctx& c, PairQueue& pq, uint64_t* gq, int32_t* gcount, HubStage& hs,
+                                                 HubItem* hout, int32_t* hn_out) {
+    int32_t v = 0, s0 = 0, nch = 0;
+    if (c.lane < hs.n) {
+        v = hs.v[c.lane];
+        s0 = hs.s[c.lane];
+        nch = (hs.d[c.lane] + kGiantChunk - 1) / kGiantChunk;
+    }
+    const int incl = wave_incl_scan(nch);
+    int qb, hb;
+    block_reserve2(c, pq.n, gcount, lane_get(incl, kWave - 1), hn_out, qb, hb);
+    for (int i = c.lane; i < pq.n; i += kWave) st_agent(&gq[qb + i], pq.buf[i]);
+    hb += incl - nch;
+    for (int k = 0; k < nch; ++k) {
+        st_agent(&hout[hb + k].v, v);
+        st_agent(&hout[hb + k].start, s0 + k * kGiantChunk);
+    }
+    pq.n = 0;
+    hs.n = 0;
+    wave_sync();
+}
+
 // Pushes a newly claimed vertex for expansion at the next level: vertices of
 // degree >= kGiantHubDeg become kGiantChunk-edge work items, others go to
 // the queue (degree-0 vertices have nothing to expand).  All lanes call it.
-__device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* __restrict__ beg, WaveQueue& wq,
-                                          int32_t* qout, int32_t* qn_out, HubStage& hs, HubItem* hout,
+__device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* __restrict__ beg, PairQueue& pq,
+                                          uint64_t* qout, int32_t* qn_out, HubStage& hs, HubItem* hout,
                                           int32_t* hn_out, int64_t& pushed_edges) {
     int32_t d = 0, s0 = 0;
     if (claim) {
@@ -343,7 +411,7 @@ __device__ __forceinline__ void push_next(bool claim, int32_t w, const int32_t* 
         pushed_edges += d;
     }
     hub_push(hs, claim && d >= kGiantHubDeg, w, s0, d, hout, hn_out);
-    wq_push<true>(wq, claim && d > 0 && d < kGiantHubDeg, w, qout, qn_out);
+    pq_push(pq, claim && d > 0 && d < kGiantHubDeg, qentry(s0, d), qout, qn_out);
 }
 
 // The acquire gsync<.., true> performs, for a block that passed a barrier
@@ -443,13 +511,13 @@ __device__ __forceinline__ void claim_with_degree(const bool (&want)[K], const i
 // push_next for K claimed targets whose offsets are known.
 template <int K>
 __device__ __forceinline__ void push_found_n(const bool (&claim)[K], const int32_t (&w)[K], const int32_t (&s0)[K],
-                                             const int32_t (&d)[K], WaveQueue& wq, int32_t* qout, int32_t* qn_out,
+                                             const int32_t (&d)[K], PairQueue& wq, uint64_t* qout, int32_t* qn_out,
                                              HubStage& hs, HubItem* hout, int32_t* hn_out, int64_t& pushed_edges) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         pushed_edges += d[k];
         hub_push(hs, claim[k] && d[k] >= kGiantHubDeg, w[k], s0[k], d[k], hout, hn_out);
-        wq_push<true>(wq, claim[k] && d[k] > 0 && d[k] < kGiantHubDeg, w[k], qout, qn_out);
+        pq_push(wq, claim[k] && d[k] > 0 && d[k] < kGiantHubDeg, qentry(s0[k], d[k]), qout, qn_out);
     }
 }
 
@@ -467,22 +535,23 @@ __device__ __forceinline__ void push_found_n(const bool (&claim)[K], const int32
 // the found counts are those of a level-synchronous traversal; only the
 // level at which a vertex is expanded changes.  All lanes call it.
 template <bool BW>
-__device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root, const int32_t* __restrict__ beg,
+__device__ __forceinline__ void expand_local(Ctx& c, PairQueue& wq, int32_t root, const int32_t* __restrict__ beg,
                                              const int32_t* __restrict__ adj, const uint64_t* __restrict__ allow,
-                                             uint64_t* vis, int32_t* qout, int32_t* qn_out, HubStage& hs,
+                                             uint64_t* vis, uint64_t* qout, int32_t* qn_out, HubStage& hs,
                                              HubItem* hout, int32_t* hn_out, int64_t& my_found, int max_steps) {
     int32_t* off = c.sm.off + c.wave * kWave;  // this wave's slice of the tile arrays
     int32_t* stv = c.sm.start + c.wave * kWave;
-    int32_t* buf = wq.buf;
-    int head = 0, n = wq.n;  // pending: buf[head .. n) modulo kWaveBuf
+    uint64_t* buf = wq.buf;
+    const int cap = wq.cap, mask = cap - 1;
+    int head = 0, n = wq.n;  // pending: buf[head .. n) modulo cap
     wave_sync();
     for (int step = 0; n > head && step < max_steps; ++step) {
         const int k = min(kWave, n - head);
         int32_t st = 0, dg = 0;
         if (c.lane < k) {
-            const int32_t v = buf[(head + c.lane) % kWaveBuf];
-            st = beg[v];
-            dg = beg[v + 1] - st;
+            const uint64_t q = buf[(head + c.lane) & mask];
+            st = qentry_start(q);
+            dg = qentry_degree(q);
         }
         head += k;
         const int incl = wave_incl_scan(dg);
@@ -509,14 +578,14 @@ __device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root
             const bool keep = claim[0] && d[0] > 0 && d[0] < kGiantHubDeg;
             const uint64_t m = __ballot(keep);
             const int cnt = __popcll(m);
-            if (cnt && n - head + cnt <= kWaveBuf) {
-                if (keep) buf[(n + __popcll(m & lanemask_lt())) % kWaveBuf] = w[0];
+            if (cnt && n - head + cnt <= cap) {
+                if (keep) buf[(n + __popcll(m & lanemask_lt())) & mask] = qentry(s0[0], d[0]);
                 n += cnt;
             } else if (cnt) {  // the queue is full: these go to the next level
                 int base = 0;
                 if (c.lane == 0) base = atomicAdd(qn_out, cnt);
                 base = lane_get(base, 0);
-                if (keep) st_agent(&qout[base + __popcll(m & lanemask_lt())], w[0]);
+                if (keep) st_agent(&qout[base + __popcll(m & lanemask_lt())], qentry(s0[0], d[0]));
             }
         }
         wave_sync();  // the queue entries and the slice are rewritten next step
@@ -526,7 +595,7 @@ __device__ __forceinline__ void expand_local(Ctx& c, WaveQueue& wq, int32_t root
         int base = 0;
         if (c.lane == 0) base = atomicAdd(qn_out, left);
         base = lane_get(base, 0);
-        for (int i = c.lane; i < left; i += kWave) st_agent(&qout[base + i], buf[(head + i) % kWaveBuf]);
+        for (int i = c.lane; i < left; i += kWave) st_agent(&qout[base + i], buf[(head + i) & mask]);
     }
     wq.n = 0;
     wave_sync();
@@ -572,13 +641,14 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
     const uint64_t* allow = a.act;
     uint64_t* vis = BW ? a.bvis : a.fvis;
     uint64_t* snap = BW ? a.snapb : a.snap;
-    int32_t* q0 = BW ? a.qb0 : a.q0;
-    int32_t* q1 = BW ? a.qb1 : a.q1;
+    uint64_t* q0 = BW ? a.qb0 : a.q0;
+    uint64_t* q1 = BW ? a.qb1 : a.q1;
     HubItem* h0 = BW ? a.hb0 : a.h0;
     HubItem* h1 = BW ? a.hb1 : a.h1;
     GiantState* gs = a.gs;
     const int dir = BW ? 1 : 0;
-    WaveQueue wq{sm.wbuf[c.wave], 0};
+    PairQueue pq{c.stage(), 0, a.pair_cap};  // top-down: staged next-level entries
+    WaveQueue wq{c.stage32(), 0};      // bottom-up: the same LDS, parked misses (ids)
     HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
     const int mode = d.mode;
     const int64_t L = d.L;
@@ -588,8 +658,8 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
     const bool multi = a.n_roots > 0;
     my_found = 0;
     fedges = 0;
-    const int32_t* qin = (L & 1) ? q1 : q0;
-    int32_t* qout = (L & 1) ? q0 : q1;
+    const uint64_t* qin = (L & 1) ? q1 : q0;
+    uint64_t* qout = (L & 1) ? q0 : q1;
     const HubItem* hin = (L & 1) ? h1 : h0;
     HubItem* hout = (L & 1) ? h0 : h1;
     int32_t* qn_out = &gs->qn[dir][(L + 1) % 3].v;
@@ -636,9 +706,9 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
                 const int64_t i = u * tile + c.tid;
                 int32_t st = 0, dg = 0;
                 if (c.tid < tile && i < n) {
-                    const int32_t v = load_agent(&qin[i]);
-                    st = beg[v];
-                    dg = beg[v + 1] - st;
+                    const uint64_t q = load_agent(&qin[i]);
+                    st = qentry_start(q);
+                    dg = qentry_degree(q);
                 }
                 int excl, total;
                 block_excl_scan<kW>(dg, excl, total, sm.tmp);
@@ -678,7 +748,7 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
                     for (int k = 0; k < kTK; ++k) {
                         my_found += claim[k];
                     }
-                    push_found_n<kTK>(claim, w, s0, d, wq, qout, qn_out, hs, hout, hn_out, fedges);
+                    push_found_n<kTK>(claim, w, s0, d, pq, qout, qn_out, hs, hout, hn_out, fedges);
                 }
                 __syncthreads();
             } else {
@@ -709,7 +779,7 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
                     for (int k = 0; k < kTK; ++k) {
                         my_found += claim[k];
                     }
-                    push_found_n<kTK>(claim, w, ws, wd, wq, qout, qn_out, hs, hout, hn_out, fedges);
+                    push_found_n<kTK>(claim, w, ws, wd, pq, qout, qn_out, hs, hout, hn_out, fedges);
                 }
             }
         }
@@ -720,9 +790,9 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
         // growing region alone: levels of up to 256-65536 entries
         // expanded so took the 4096^2 grid from 5.09 to 5.8-6.5 ms.
         if (a.td_local > 0 && (mode == 2 || (L > 0 && nh == 0 && n >= kLocalPerWave * total_waves)))
-            expand_local<BW>(c, wq, root, beg, adj, allow, vis, qout, qn_out, hs, hout, hn_out, my_found,
+            expand_local<BW>(c, pq, root, beg, adj, allow, vis, qout, qn_out, hs, hout, hn_out, my_found,
                              a.td_local);
-        block_flush_both(c, wq, qout, qn_out, hs, hout, hn_out);
+        block_flush_both(c, pq, qout, qn_out, hs, hout, hn_out);
     } else {
         // ---- bottom-up over the candidate bitmap, one wave per group of
         // kU consecutive 64-vertex words; only that wave writes those
@@ -901,11 +971,11 @@ __device__ __forceinline__ void gather_finds(Ctx& c, const Dir& d) {
     const int32_t* beg = BW ? a.bwb : a.fwb;
     const uint64_t* vis = BW ? a.bvis : a.fvis;
     const uint64_t* snap = BW ? a.snapb : a.snap;
-    int32_t* q_next = (d.L & 1) ? (BW ? a.qb1 : a.q1) : (BW ? a.qb0 : a.q0);
+    uint64_t* q_next = (d.L & 1) ? (BW ? a.qb1 : a.q1) : (BW ? a.qb0 : a.q0);
     HubItem* h_next = (d.L & 1) ? (BW ? a.hb1 : a.h1) : (BW ? a.hb0 : a.h0);
     int32_t* qn_next = &gs->qn[dir][d.L % 3].v;
     int32_t* hn_next = &gs->hn[dir][d.L % 3].v;
-    WaveQueue wq{sm.wbuf[c.wave], 0};
+    PairQueue wq{c.stage(), 0, a.pair_cap};
     HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
     const int64_t total_waves = (int64_t)c.G * kW;
     const int64_t gwave = (int64_t)c.b * kW + c.wave;
@@ -948,11 +1018,11 @@ __device__ __forceinline__ bool traverse_joint(Ctx& c, int32_t root, int64_t set
         // visited, queued or split into hub items like any claimed vertex
         int64_t seeded = 0;
         for (int dir = 0; dir < 2; ++dir) {
-            WaveQueue wq{sm.wbuf[c.wave], 0};
+            PairQueue wq{c.stage(), 0, a.pair_cap};
             HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
             const int32_t* beg = dir ? a.bwb : a.fwb;
             uint64_t* vis = dir ? a.bvis : a.fvis;
-            int32_t* q0 = dir ? a.qb0 : a.q0;
+            uint64_t* q0 = dir ? a.qb0 : a.q0;
             HubItem* h0 = dir ? a.hb0 : a.h0;
             int64_t unused = 0;
             for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < a.n_roots; ib += (int64_t)c.G * kGBlock) {
@@ -1095,7 +1165,7 @@ __device__ __forceinline__ bool traverse_joint(Ctx& c, int32_t root, int64_t set
 __device__ __forceinline__ int64_t trim_sweep(Ctx& c, const int32_t* fin, int64_t n, int32_t* fout, int32_t* fcount,
                                               int64_t lo = -1) {
     const GiantArgs& a = c.a;
-    WaveQueue wq{c.sm.wbuf[c.wave], 0};
+    WaveQueue wq{c.stage32(), 0};
     const uint64_t* act = a.act;
     auto live = [act](int32_t w, int32_t self) { return w != self && bit_of(act, w); };
     int trimmed = 0, scanned = 0;
@@ -1297,7 +1367,7 @@ __device__ __forceinline__ bool colour_scc(Ctx& c, int& cur, int64_t n, int s, i
         }
         if (!gsync(gs, c.G, c.sm, a.dbg)) return false;
         // label the SCCs found, take them out of the active set, queue the rest
-        WaveQueue wq{c.sm.wbuf[c.wave], 0};
+        WaveQueue wq{c.stage32(), 0};
         int found = 0;
         for (int64_t ib = (int64_t)c.b * kGBlock + c.wave * kWave; ib < n; ib += nthreads) {
             const int64_t i = ib + c.lane;
@@ -1399,7 +1469,7 @@ __device__ __forceinline__ bool rest_phase(Ctx& c, int64_t fq0, int64_t n, int64
         // would walk it entry by entry): those are queued in the wave's LDS
         // buffer and checked afterwards by the whole wave, 64 entries a step.
         {
-            int32_t* lq = c.sm.wbuf[c.wave];
+            int32_t* lq = c.stage32();
             int32_t& nlq = c.sm.wcnt[0][c.wave];
             if (c.lane == 0) nlq = 0;
             wave_sync();
@@ -1896,11 +1966,25 @@ __global__ void k_first(const int32_t* __restrict__ beg, const int32_t* __restri
         first[v] = beg[v + 1] > beg[v] ? adj[beg[v]] : -1;
 }
 
+size_t giant_stage_bytes(int pair_cap) { return (size_t)kW * pair_cap * 8; }
+
+// Low-degree graphs (lattices, road-like meshes) run long top-down
+// traversals that a wave expands in place, and a ring of 512 entries keeps
+// more of them in the wave than 256 (grid 4096: 4.41 -> 4.11 ms); R-MAT at
+// degree ~16 times the same with either and keeps the smaller one.
+int giant_pair_cap(double avg_degree) { return avg_degree < 8 ? 512 : 256; }
+
 int giant_max_blocks_per_cu() {
+    // the grid must fit with the largest staging any launch asks for
+    const size_t most = giant_stage_bytes(kPairCapMax);
+    for (const void* k : {reinterpret_cast<const void*>(k_giant), reinterpret_cast<const void*>(k_tail)})
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) != hipSuccess) return 0;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kGBlock, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kGBlock, most) != hipSuccess) return 0;
     return nb;
 }
+
+static bool pair_cap_ok(int c) { return c >= kPairCapMin && c <= kPairCapMax && (c & (c - 1)) == 0; }
 
 int64_t giant_bu_mult(int grid) {
     const int64_t t = (int64_t)grid * kW;
@@ -1927,14 +2011,18 @@ hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, h
     copy.epoch = ++epochs;
     copy.state_words = (int32_t)((a.trace ? offsetof(GiantState, epoch) : kGiantStateHead) / 8);
     copy.ctl_words = (int32_t)(sizeof(Ctl) / 8);
+    if (!pair_cap_ok(a.pair_cap)) return hipErrorInvalidValue;
     void* args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args, 0, s);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args,
+                                      giant_stage_bytes(a.pair_cap), s);
 }
 
 hipError_t launch_tail(const GiantArgs& a, int grid, hipStream_t s) {
+    if (!pair_cap_ok(a.pair_cap)) return hipErrorInvalidValue;
     GiantArgs copy = a;
     void* args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tail), dim3(grid), dim3(kGBlock), args, 0, s);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tail), dim3(grid), dim3(kGBlock), args,
+                                      giant_stage_bytes(a.pair_cap), s);
 }
 
 hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s) {

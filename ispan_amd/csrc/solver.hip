@@ -239,8 +239,10 @@ int ensure_buffers(ispan_ctx* ctx, int32_t V, int64_t E) {
     CK(hipMalloc(&ctx->fvis, nw * 8));
     CK(hipMalloc(&ctx->bvis, nw * 8));
     for (int i = 0; i < 2; ++i) {
-        CK(hipMalloc(&ctx->fq[i], nv * 4));
-        CK(hipMalloc(&ctx->q[i], nv * 4));
+        // 8 bytes a vertex: the giant kernel's top-down queue entries
+        // (first edge, degree); the other phases queue 4-byte ids in them
+        CK(hipMalloc(&ctx->fq[i], nv * 8));
+        CK(hipMalloc(&ctx->q[i], nv * 8));
         CK(hipMalloc(&ctx->hub[i], (size_t)hub_capacity(E) * sizeof(HubItem)));
     }
     CK(hipMalloc(&ctx->parent, nv * 4));
@@ -456,15 +458,15 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.fvis = ctx->fvis;
         ga.bvis = ctx->bvis;
         ga.snap = ctx->snap;
-        ga.q0 = ctx->q[0];
-        ga.q1 = ctx->q[1];
+        ga.q0 = reinterpret_cast<uint64_t*>(ctx->q[0]);
+        ga.q1 = reinterpret_cast<uint64_t*>(ctx->q[1]);
         ga.h0 = ctx->gh[0];
         ga.h1 = ctx->gh[1];
         ga.snapb = ctx->snapb;
         // the backward direction runs beside the forward one: its queues are
         // the active-queue buffers, idle until the post phase
-        ga.qb0 = ctx->fq[0];
-        ga.qb1 = ctx->fq[1];
+        ga.qb0 = reinterpret_cast<uint64_t*>(ctx->fq[0]);
+        ga.qb1 = reinterpret_cast<uint64_t*>(ctx->fq[1]);
         ga.hb0 = ctx->ghb[0];
         ga.hb1 = ctx->ghb[1];
         ga.fq = ctx->fq[1];
@@ -476,6 +478,7 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.beta = P->beta;
         ga.gamma = P->gamma;
         ga.avg_degree = avg_degree;
+        ga.pair_cap = giant_pair_cap(avg_degree);
         ga.wrap32 = P->pivot_wrap32;
         ga.switch_rule = P->switch_rule;
         ga.roots = ctx->batch_roots;
@@ -1760,8 +1763,8 @@ size_t carve_solve_state(ispan_ctx* c, unsigned char* base, int32_t V, int64_t E
     take(c->fvis, nw * 8);
     take(c->bvis, nw * 8);
     for (int i = 0; i < 2; ++i) {
-        take(c->fq[i], nv * 4);
-        take(c->q[i], nv * 4);
+        take(c->fq[i], nv * 8);
+        take(c->q[i], nv * 8);
         take(c->hub[i], (size_t)hub_capacity(E) * sizeof(HubItem));
     }
     take(c->parent, nv * 4);

@@ -76,9 +76,9 @@ def test_stale_scratch_and_caller_stream():
         s.destroy()
 
 
-@pytest.mark.parametrize("scale,V", [(19, None), (20, 900_000)])
+@pytest.mark.parametrize("scale,V", [(19, None), (20, 750_000)])
 def test_largest_documented_size_fits(scale, V):
-    """The sizes the header names: R-MAT scale 19 and 900,000 vertices at
+    """The sizes the header names: R-MAT scale 19 and 750,000 vertices at
     edge factor 16."""
     g = O.rmat(scale, 16, 2, V=V)
     case(f"R-MAT s{scale} V={g.V} E={g.E}")
