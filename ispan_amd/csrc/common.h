@@ -174,8 +174,6 @@ struct GiantArgs {
     int32_t td_local;         // top-down to exhaustion: steps a wave expands its own finds in place (0: off)
     uint64_t* ckey;           // colouring: (priority, colour) per vertex
     int32_t* aux;             // colouring: smallest member per SCC root
-    int32_t pair_cap;         // per-wave LDS staging, in 8-byte entries: a power of two in
-                              // [kPairCapMin, kPairCapMax] (giant_pair_cap chooses it)
     // set by launch_giant: block 0 clears `state_words` words of gs and
     // `ctl_words` of ctl, then publishes `epoch` (unique per launch, never 0)
     // in gs->epoch; the other blocks start once they read it
@@ -191,13 +189,7 @@ constexpr int kGiantChunk = 8192;   // edges per work item
 // d / kGiantChunk + 1 items, and at most E / kGiantHubDeg vertices qualify.
 inline int64_t giant_hub_capacity(int64_t E) { return E / kGiantChunk + E / kGiantHubDeg + 64; }
 
-// Per-wave LDS staging of the giant kernel, in 8-byte entries: the top-down
-// queue entries a wave stages and the ring of its in-place expansion.
-constexpr int kPairCapMin = 256;
-constexpr int kPairCapMax = 1024;
-// GiantArgs::pair_cap for a graph of this average degree
-int giant_pair_cap(double avg_degree);
-// with the largest staging (kPairCapMax); 0 on error
+// 0 on error
 int giant_max_blocks_per_cu();
 // GiantArgs::bu_mult for a grid of `grid` blocks
 int64_t giant_bu_mult(int grid);

@@ -49,11 +49,13 @@ struct Smem {
     int32_t hd[kW][kWave];
 };
 
-// Per-wave staging in dynamic LDS, GiantArgs::pair_cap 8-byte entries a wave:
-// top-down queue entries, or kWaveBuf vertex ids in the phases that queue
-// ids.  The capacity is chosen per launch (giant_pair_cap): only low-degree
-// graphs gain from the larger one.
-static_assert(kWaveBuf * 4 <= kPairCapMin * 8, "the id view fits the smallest staging");
+// Per-wave staging, kPairCap 8-byte entries a wave: top-down queue entries,
+// or kWaveBuf vertex ids in the phases that queue ids.  It is dynamic LDS,
+// placed after Smem, so that Smem's arrays stay within the 64 KB an LDS
+// instruction's immediate offset reaches.
+constexpr int kPairCap = 512;
+static_assert(kWaveBuf * 4 <= kPairCap * 8, "the id view fits the staging");
+static_assert((kPairCap & (kPairCap - 1)) == 0, "the in-place ring indexes with a mask");
 extern __shared__ uint64_t stage_lds[];
 
 struct Ctx {
@@ -61,8 +63,8 @@ struct Ctx {
     int G, b, tid, lane, wave;
     Smem& sm;
     int64_t edges_td, edges_bu, found_td, found_bu;  // work counters (thread 0's are the block's)
-    // this wave's staging (a.pair_cap entries), as 8-byte entries or as ids
-    __device__ uint64_t* stage() const { return stage_lds + wave * a.pair_cap; }
+    // this wave's staging, as 8-byte entries or as ids
+    __device__ uint64_t* stage() const { return stage_lds + wave * kPairCap; }
     __device__ int32_t* stage32() const { return reinterpret_cast<int32_t*>(stage()); }
 };
 
@@ -351,8 +353,7 @@ __device__ __forceinline__ int32_t qentry_degree(uint64_t e) { return (int32_t)(
 // A wave's staged top-down queue entries (in LDS) and its fill.
 struct PairQueue {
     uint64_t* buf;
-    int n;    // wave-uniform
-    int cap;  // a power of two
+    int n;  // wave-uniform
 };
 
 __device__ __forceinline__ void pq_flush(PairQueue& pq, uint64_t* gq, int32_t* gcount) {
@@ -370,7 +371,7 @@ __device__ __forceinline__ void pq_push(PairQueue& pq, bool pred, uint64_t e, ui
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
     const int c = __popcll(m);
-    if (pq.n + c > pq.cap) pq_flush(pq, gq, gcount);
+    if (pq.n + c > kPairCap) pq_flush(pq, gq, gcount);
     if (pred) pq.buf[pq.n + __popcll(m & lanemask_lt())] = e;
     pq.n += c;
 }
@@ -542,8 +543,8 @@ __device__ __forceinline__ void expand_local(Ctx& c, PairQueue& wq, int32_t root
     int32_t* off = c.sm.off + c.wave * kWave;  // this wave's slice of the tile arrays
     int32_t* stv = c.sm.start + c.wave * kWave;
     uint64_t* buf = wq.buf;
-    const int cap = wq.cap, mask = cap - 1;
-    int head = 0, n = wq.n;  // pending: buf[head .. n) modulo cap
+    constexpr int mask = kPairCap - 1;
+    int head = 0, n = wq.n;  // pending: buf[head .. n) modulo kPairCap
     wave_sync();
     for (int step = 0; n > head && step < max_steps; ++step) {
         const int k = min(kWave, n - head);
@@ -578,7 +579,7 @@ __device__ __forceinline__ void expand_local(Ctx& c, PairQueue& wq, int32_t root
             const bool keep = claim[0] && d[0] > 0 && d[0] < kGiantHubDeg;
             const uint64_t m = __ballot(keep);
             const int cnt = __popcll(m);
-            if (cnt && n - head + cnt <= cap) {
+            if (cnt && n - head + cnt <= kPairCap) {
                 if (keep) buf[(n + __popcll(m & lanemask_lt())) & mask] = qentry(s0[0], d[0]);
                 n += cnt;
             } else if (cnt) {  // the queue is full: these go to the next level
@@ -647,7 +648,7 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
     HubItem* h1 = BW ? a.hb1 : a.h1;
     GiantState* gs = a.gs;
     const int dir = BW ? 1 : 0;
-    PairQueue pq{c.stage(), 0, a.pair_cap};  // top-down: staged next-level entries
+    PairQueue pq{c.stage(), 0};  // top-down: staged next-level entries
     WaveQueue wq{c.stage32(), 0};      // bottom-up: the same LDS, parked misses (ids)
     HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
     const int mode = d.mode;
@@ -975,7 +976,7 @@ __device__ __forceinline__ void gather_finds(Ctx& c, const Dir& d) {
     HubItem* h_next = (d.L & 1) ? (BW ? a.hb1 : a.h1) : (BW ? a.hb0 : a.h0);
     int32_t* qn_next = &gs->qn[dir][d.L % 3].v;
     int32_t* hn_next = &gs->hn[dir][d.L % 3].v;
-    PairQueue wq{c.stage(), 0, a.pair_cap};
+    PairQueue wq{c.stage(), 0};
     HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
     const int64_t total_waves = (int64_t)c.G * kW;
     const int64_t gwave = (int64_t)c.b * kW + c.wave;
@@ -1018,7 +1019,7 @@ __device__ __forceinline__ bool traverse_joint(Ctx& c, int32_t root, int64_t set
         // visited, queued or split into hub items like any claimed vertex
         int64_t seeded = 0;
         for (int dir = 0; dir < 2; ++dir) {
-            PairQueue wq{c.stage(), 0, a.pair_cap};
+            PairQueue wq{c.stage(), 0};
             HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
             const int32_t* beg = dir ? a.bwb : a.fwb;
             uint64_t* vis = dir ? a.bvis : a.fvis;
@@ -1966,25 +1967,16 @@ __global__ void k_first(const int32_t* __restrict__ beg, const int32_t* __restri
         first[v] = beg[v + 1] > beg[v] ? adj[beg[v]] : -1;
 }
 
-size_t giant_stage_bytes(int pair_cap) { return (size_t)kW * pair_cap * 8; }
-
-// Low-degree graphs (lattices, road-like meshes) run long top-down
-// traversals that a wave expands in place, and a ring of 512 entries keeps
-// more of them in the wave than 256 (grid 4096: 4.41 -> 4.11 ms); R-MAT at
-// degree ~16 times the same with either and keeps the smaller one.
-int giant_pair_cap(double avg_degree) { return avg_degree < 8 ? 512 : 256; }
+constexpr size_t kStageBytes = (size_t)kW * kPairCap * 8;  // dynamic LDS of k_giant and k_tail
 
 int giant_max_blocks_per_cu() {
-    // the grid must fit with the largest staging any launch asks for
-    const size_t most = giant_stage_bytes(kPairCapMax);
     for (const void* k : {reinterpret_cast<const void*>(k_giant), reinterpret_cast<const void*>(k_tail)})
-        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) != hipSuccess) return 0;
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageBytes) != hipSuccess)
+            return 0;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kGBlock, most) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_giant, kGBlock, kStageBytes) != hipSuccess) return 0;
     return nb;
 }
-
-static bool pair_cap_ok(int c) { return c >= kPairCapMin && c <= kPairCapMax && (c & (c - 1)) == 0; }
 
 int64_t giant_bu_mult(int grid) {
     const int64_t t = (int64_t)grid * kW;
@@ -2011,18 +2003,16 @@ hipError_t launch_giant(const GiantArgs& a, Ctl* ctl, int grid, hipStream_t s, h
     copy.epoch = ++epochs;
     copy.state_words = (int32_t)((a.trace ? offsetof(GiantState, epoch) : kGiantStateHead) / 8);
     copy.ctl_words = (int32_t)(sizeof(Ctl) / 8);
-    if (!pair_cap_ok(a.pair_cap)) return hipErrorInvalidValue;
     void* args[] = {&copy};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_giant), dim3(grid), dim3(kGBlock), args,
-                                      giant_stage_bytes(a.pair_cap), s);
+                                      kStageBytes, s);
 }
 
 hipError_t launch_tail(const GiantArgs& a, int grid, hipStream_t s) {
-    if (!pair_cap_ok(a.pair_cap)) return hipErrorInvalidValue;
     GiantArgs copy = a;
     void* args[] = {&copy};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tail), dim3(grid), dim3(kGBlock), args,
-                                      giant_stage_bytes(a.pair_cap), s);
+                                      kStageBytes, s);
 }
 
 hipError_t launch_first(const int32_t* beg, const int32_t* adj, int32_t V, int32_t* first, int grid, hipStream_t s) {

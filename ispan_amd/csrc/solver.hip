@@ -478,7 +478,6 @@ int solve_impl(ispan_ctx* ctx, const ispan_params* P, int32_t* d_labels, ispan_s
         ga.beta = P->beta;
         ga.gamma = P->gamma;
         ga.avg_degree = avg_degree;
-        ga.pair_cap = giant_pair_cap(avg_degree);
         ga.wrap32 = P->pivot_wrap32;
         ga.switch_rule = P->switch_rule;
         ga.roots = ctx->batch_roots;
