@@ -28,6 +28,25 @@ __device__ __forceinline__ void st_agent(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// st_agent of four consecutive int32s (p 16-byte aligned) in one 16-byte
+// store: the same write-through store (global_store_dwordx4 with sc1, as
+// st_agent's global_store_dword), so a wave writes 1 KB per instruction.
+// A VALU write to the data registers of a store wider than 8 bytes in the
+// next cycle needs one wait state, which the compiler inserts for its own
+// stores but not after inline asm: hence the s_nop.
+__device__ __forceinline__ void st_agent4(int32_t* p, int32_t x, int32_t y, int32_t z, int32_t w) {
+    typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 v = {x, y, z, w};
+    asm volatile("global_store_dwordx4 %0, %1, off sc1\n\ts_nop 0" : : "v"(p), "v"(v) : "memory");
+}
+
+// v as held by lane src (any lane index per lane: ds_bpermute).
+__device__ __forceinline__ uint64_t lane_from(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int32_t)(uint32_t)v, src, kWave);
+    const uint32_t hi = (uint32_t)__shfl((int32_t)(uint32_t)(v >> 32), src, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Lanes of one wave exchange data through memory: order and publish it.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

@@ -1816,8 +1816,8 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
     }
     if (a.trace && c.tid == 0) atomicMax((unsigned long long*)&gs->post_cp[1], (unsigned long long)wall_clock64());
     // the labels: one word per lane, the wave loads 64 words at once (its own
-    // block's, narrowed above), then writes their labels word by word
-    // (coalesced 256 B stores, no loads in the loop)
+    // block's, narrowed above), then writes them four words at a time, four
+    // consecutive labels a lane (coalesced 1 KB stores, no loads in the loop)
     for (int64_t base = wb + (int64_t)c.wave * kWave; base < we; base += (int64_t)kW * kWave) {
         const int64_t w = base + c.lane;
         uint64_t ac = 0, g = 0, sw = 0;
@@ -1827,12 +1827,27 @@ __global__ __launch_bounds__(kGBlock) void k_giant(const GiantArgs a) {
             sw = a.sel ? a.sel[w] : ~0ull;  // unselected vertices keep whatever label they had
         }
         const int nw = (int)min<int64_t>(kWave, we - base);
-        for (int j = 0; j < nw; ++j) {
-            const uint64_t aj = lane_get(ac, j), gj = lane_get(g, j), sj = lane_get(sw, j);
-            const int64_t v = (base + j) * 64 + c.lane;
-            if (v < a.V && ((sj >> c.lane) & 1ull)) {
-                const bool gi = (gj >> c.lane) & 1ull, act_v = (aj >> c.lane) & 1ull;
-                st_agent(&a.lab[v], gi ? (multi ? (int32_t)v : gmin) : (act_v ? -1 : (int32_t)v));
+        const int q = (c.lane & 15) * 4;  // this lane's first bit in its word
+        for (int j = 0; j < nw; j += 4) {
+            const int src = j + (c.lane >> 4);  // this lane's word in the group of four
+            const uint64_t aj = lane_from(ac, src), gj = lane_from(g, src), sj = lane_from(sw, src);
+            if (src >= nw) continue;
+            const int64_t v0 = (base + src) * 64 + q;
+            int32_t l[4];
+            bool all = v0 + 3 < a.V;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int64_t v = v0 + k;
+                const bool gi = (gj >> (q + k)) & 1ull, act_v = (aj >> (q + k)) & 1ull;
+                l[k] = gi ? (multi ? (int32_t)v : gmin) : (act_v ? -1 : (int32_t)v);
+                all = all && ((sj >> (q + k)) & 1ull);
+            }
+            if (all) {
+                st_agent4(&a.lab[v0], l[0], l[1], l[2], l[3]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (v0 + k < a.V && ((sj >> (q + k)) & 1ull)) st_agent(&a.lab[v0 + k], l[k]);
             }
         }
     }
