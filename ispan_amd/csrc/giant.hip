@@ -603,8 +603,10 @@ __device__ __forceinline__ void expand_local(Ctx& c, PairQueue& wq, int32_t root
 }
 
 // Level-synchronous top-down levels are expanded in place (as in mode 2)
-// once their input gives every wave at least this many vertices.
-constexpr int kLocalPerWave = 16;
+// once their input gives every wave at least this many vertices (grid 4096:
+// 2, 4, 8, 16 and 32 measured 4.30, 4.19, 4.01, 4.09 and 4.30 ms; R-MAT
+// flat).
+constexpr int kLocalPerWave = 8;
 
 // Top-down: edges per lane per step.  One measured fastest: two let their
 // load chains overlap, but the registers they hold push the kernel into
