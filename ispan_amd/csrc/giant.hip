@@ -650,8 +650,8 @@ __device__ __forceinline__ void level_work(Ctx& c, const Dir& d, int32_t root, i
     HubItem* h1 = BW ? a.hb1 : a.h1;
     GiantState* gs = a.gs;
     const int dir = BW ? 1 : 0;
-    PairQueue pq{c.stage(), 0};  // top-down: staged next-level entries
-    WaveQueue wq{c.stage32(), 0};      // bottom-up: the same LDS, parked misses (ids)
+    PairQueue pq{c.stage(), 0};    // top-down: staged next-level entries
+    WaveQueue wq{c.stage32(), 0};  // bottom-up: the same LDS, parked misses (ids)
     HubStage hs{sm.hv[c.wave], sm.hs[c.wave], sm.hd[c.wave], 0};
     const int mode = d.mode;
     const int64_t L = d.L;
